@@ -1,0 +1,125 @@
+/* CPU oracle for the COFFEE force-free field-update hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  The product never links or calls it.
+ *
+ * It restates, in plain C with OpenMP over k, the algorithms of:
+ *   flat EZ   src/algorithms/field_solver_EZ.cpp (CPU semantics, use_cuda=0),
+ *             src/algorithms/damping_boundary.cpp, src/algorithms/pulsar_impl.hpp,
+ *             src/sim_env.cpp:365-438 (single-rank periodic guard fill);
+ *   spherical src/algorithms/field_solver_EZ_spherical.cu, metric_sph.h;
+ *   GR        src/algorithms/field_solver_gr_EZ.cu, boundary.cu, metric_cks_impl.hpp.
+ * Every function cites the reference lines it follows.
+ *
+ * Pinning status (what checks each function) is listed in the header comment
+ * of each .c file.
+ *
+ * Arrays are Fortran order with guards: idx = i + (j + k*dims1)*dims0
+ * (src/data/multi_array_impl.hpp:79-91).  Field order in the 7-array state
+ * U is Ex,Ey,Ez,Bx,By,Bz,P.
+ */
+#ifndef FFE_ORACLE_H
+#define FFE_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Subset of Coffee::sim_params (src/sim_params.h:15-124) that the hot path
+ * reads, held in double/int; each solver rounds to its Scalar on entry exactly
+ * as the reference's parse_config stores params in Scalar. */
+typedef struct ffe_params {
+  int N[3];
+  int guard[3];
+  double lower[3];
+  double size[3];
+  int periodic[3];
+  double dt;
+  int pml[3];
+  int pmllen;
+  double sigpml;
+  int divB_clean;
+  double ch2;
+  double tau;
+  double KOeps;
+  int KO_geometry;
+  int use_edotb_damping;
+  double damp_gamma;
+  int calc_current;
+  int clean_ep;
+  int check_egb;
+  int problem;
+  double radius, omega, b0;
+  double p1, p2, p3;
+  double q11, q12, q13, q22, q23;
+  double q_offset_x, q_offset_y, q_offset_z;
+  double tp_start, tp_end, rpert1, rpert2, dw0, nT, ph1, ph2, dph, theta0, drpert;
+  double tp_start1, tp_end1, rpert11, rpert21, dw1, nT1, ph11, ph21, dph1;
+  int pert_type;
+  double a;
+  /* SIMD width of the reference CPU build whose tanh-mask positions are
+   * reproduced (field_solver_EZ.cpp:222-224 adds the lane offset to x, y and
+   * z alike); 1 evaluates the mask at the node itself. */
+  int mask_lanes;
+} ffe_params;
+
+/* Optional per-cell diagnostics written by the flat solver, as the reference
+ * does every stage (field_solver_EZ.cpp:123-125,149-150,198-200) and in its
+ * energy accounting (.cpp:447-453,588-623).  Any pointer may be NULL. */
+typedef struct ffe_diag_f64 {
+  double *J[3];
+  double *divE, *divB;
+  double *dU_KO, *dU_Epar, *dU_EgtB;
+  double *dU_KO_cum, *dU_Epar_cum, *dU_EgtB_cum;
+} ffe_diag_f64;
+
+typedef struct ffe_diag_f32 {
+  float *J[3];
+  float *divE, *divB;
+  float *dU_KO, *dU_Epar, *dU_EgtB;
+  float *dU_KO_cum, *dU_Epar_cum, *dU_EgtB_cum;
+} ffe_diag_f32;
+
+/* ---- flat Cartesian EZ (field_solver_EZ.cpp) ---------------------------- */
+
+/* Advance nsteps steps from `time`.  U: 7 state arrays; dU: 7 LSRK registers
+ * (dE, dB, dP); Un: 7 step-start copies (Etmp, Btmp, Ptmp; also KO scratch);
+ * Bbg: 3 absorbing-layer background arrays (may be NULL when pml==0). */
+int ffe_flat_evolve_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                        double *const Un[7], double *const Bbg[3], const ffe_diag_f64 *diag,
+                        double time, int nsteps);
+int ffe_flat_evolve_f32(const ffe_params *p, float *const U[7], float *const dU[7],
+                        float *const Un[7], float *const Bbg[3], const ffe_diag_f32 *diag,
+                        double time, int nsteps);
+
+/* Building blocks exposed for unit tests (single calls, interior only). */
+int ffe_flat_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                          const ffe_diag_f64 *diag, double As, double Bs);
+int ffe_flat_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7],
+                    const ffe_diag_f64 *diag);
+int ffe_flat_project_f64(const ffe_params *p, double *const U[7], const ffe_diag_f64 *diag);
+int ffe_flat_fill_guards_f64(const ffe_params *p, double *const U[7]);
+int ffe_flat_absorb_f64(const ffe_params *p, double *const U[7], double *const Un[7],
+                        double *const Bbg[3]);
+int ffe_flat_boundary_f64(const ffe_params *p, double *const U[7], double t);
+/* Bbg / initial dipole: b0 * quadru_dipole(x,y,z,...,phase=0,n) on collocated
+ * nodes including guards (field_solver_EZ.cpp:66-85). */
+int ffe_flat_bbg_f64(const ffe_params *p, double *const Bbg[3]);
+int ffe_flat_bbg_f32(const ffe_params *p, float *const Bbg[3]);
+
+/* Closed-form helpers (src/algorithms/pulsar_impl.hpp) in double. */
+double ffe_dipole2(double x, double y, double z, double p1, double p2, double p3,
+                   double phase, int n);
+double ffe_quadru_dipole(const ffe_params *p, double x, double y, double z, double phase,
+                         int n);
+double ffe_diff1_4(const double *f, long ijk, long s);
+double ffe_diff6_2(const double *f, long ijk, long s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,210 @@
+"""ctypes access to the CPU oracle and the reference-build harness.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports
+this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import shutil
+import subprocess
+import tempfile
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+class FFEParams(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int * 3), ("guard", ctypes.c_int * 3),
+        ("lower", ctypes.c_double * 3), ("size", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3), ("dt", ctypes.c_double),
+        ("pml", ctypes.c_int * 3), ("pmllen", ctypes.c_int), ("sigpml", ctypes.c_double),
+        ("divB_clean", ctypes.c_int), ("ch2", ctypes.c_double), ("tau", ctypes.c_double),
+        ("KOeps", ctypes.c_double), ("KO_geometry", ctypes.c_int),
+        ("use_edotb_damping", ctypes.c_int), ("damp_gamma", ctypes.c_double),
+        ("calc_current", ctypes.c_int), ("clean_ep", ctypes.c_int), ("check_egb", ctypes.c_int),
+        ("problem", ctypes.c_int),
+        ("radius", ctypes.c_double), ("omega", ctypes.c_double), ("b0", ctypes.c_double),
+        ("p1", ctypes.c_double), ("p2", ctypes.c_double), ("p3", ctypes.c_double),
+        ("q11", ctypes.c_double), ("q12", ctypes.c_double), ("q13", ctypes.c_double),
+        ("q22", ctypes.c_double), ("q23", ctypes.c_double),
+        ("q_offset_x", ctypes.c_double), ("q_offset_y", ctypes.c_double),
+        ("q_offset_z", ctypes.c_double),
+        ("tp_start", ctypes.c_double), ("tp_end", ctypes.c_double), ("rpert1", ctypes.c_double),
+        ("rpert2", ctypes.c_double), ("dw0", ctypes.c_double), ("nT", ctypes.c_double),
+        ("ph1", ctypes.c_double), ("ph2", ctypes.c_double), ("dph", ctypes.c_double),
+        ("theta0", ctypes.c_double), ("drpert", ctypes.c_double),
+        ("tp_start1", ctypes.c_double), ("tp_end1", ctypes.c_double),
+        ("rpert11", ctypes.c_double), ("rpert21", ctypes.c_double), ("dw1", ctypes.c_double),
+        ("nT1", ctypes.c_double), ("ph11", ctypes.c_double), ("ph21", ctypes.c_double),
+        ("dph1", ctypes.c_double),
+        ("pert_type", ctypes.c_int), ("a", ctypes.c_double), ("mask_lanes", ctypes.c_int),
+    ]
+
+
+def make_params(sp, mask_lanes: int = 1) -> FFEParams:
+    """FFEParams from a coffeegpu_amd.params.SimParams."""
+    p = FFEParams()
+    for i in range(3):
+        p.N[i] = int(sp.N[i])
+        p.guard[i] = int(sp.guard[i])
+        p.lower[i] = float(sp.lower[i])
+        p.size[i] = float(sp.size[i])
+        p.periodic[i] = int(bool(sp.periodic_boundary[i]))
+        p.pml[i] = int(sp.pml[i])
+    for name, _ in FFEParams._fields_:
+        if name in ("N", "guard", "lower", "size", "periodic", "pml", "mask_lanes"):
+            continue
+        v = getattr(sp, name)
+        setattr(p, name, int(v) if isinstance(v, (bool, int)) and not isinstance(v, float) else v)
+    p.mask_lanes = int(mask_lanes)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _ptrs(arrs: Sequence[np.ndarray], ctype):
+    for a in arrs:
+        assert a.flags["C_CONTIGUOUS"]
+    P = ctypes.POINTER(ctype)
+    return (P * len(arrs))(*[a.ctypes.data_as(P) for a in arrs])
+
+
+class FlatState:
+    """Host state of the flat solver: U, dU, Un (7 each), Bbg (3)."""
+
+    def __init__(self, U: np.ndarray, dtype, bbg: Optional[np.ndarray] = None):
+        self.dtype = np.dtype(dtype)
+        self.U = np.ascontiguousarray(U, dtype=self.dtype).copy()
+        self.dU = np.zeros_like(self.U)
+        self.Un = np.zeros_like(self.U)
+        self.Bbg = None if bbg is None else np.ascontiguousarray(bbg, dtype=self.dtype).copy()
+
+
+def flat_evolve(sp, state: FlatState, nsteps: int, time: float = 0.0, mask_lanes: int = 1,
+                diag: Optional[Dict[str, np.ndarray]] = None):
+    L = lib()
+    p = make_params(sp, mask_lanes)
+    ct = ctypes.c_double if state.dtype == np.float64 else ctypes.c_float
+    U = _ptrs(list(state.U), ct)
+    dU = _ptrs(list(state.dU), ct)
+    Un = _ptrs(list(state.Un), ct)
+    bbg = _ptrs(list(state.Bbg), ct) if state.Bbg is not None else None
+    fn = L.ffe_flat_evolve_f64 if state.dtype == np.float64 else L.ffe_flat_evolve_f32
+    fn.restype = ctypes.c_int
+    dg = None
+    if diag is not None:
+        dg = _make_diag(diag, ct, state.dtype)
+    rc = fn(ctypes.byref(p), U, dU, Un, bbg, ctypes.byref(dg) if dg is not None else None,
+            ctypes.c_double(time), ctypes.c_int(nsteps))
+    if rc != 0:
+        raise RuntimeError(f"oracle flat_evolve failed rc={rc}")
+    return state
+
+
+def _make_diag(d, ct, dtype):
+    P = ctypes.POINTER(ct)
+
+    class Diag(ctypes.Structure):
+        _fields_ = [("J", P * 3), ("divE", P), ("divB", P), ("dU_KO", P), ("dU_Epar", P),
+                    ("dU_EgtB", P), ("dU_KO_cum", P), ("dU_Epar_cum", P), ("dU_EgtB_cum", P)]
+
+    g = Diag()
+    if "J" in d:
+        for i in range(3):
+            g.J[i] = d["J"][i].ctypes.data_as(P)
+    for k in ("divE", "divB", "dU_KO", "dU_Epar", "dU_EgtB", "dU_KO_cum", "dU_Epar_cum",
+              "dU_EgtB_cum"):
+        if k in d:
+            setattr(g, k, d[k].ctypes.data_as(P))
+    return g
+
+
+def flat_bbg(sp, dtype) -> np.ndarray:
+    from coffeegpu_amd.params import Grid
+
+    g = Grid.from_params(sp, dtype)
+    out = np.zeros((3,) + g.shape, dtype=dtype)
+    ct = ctypes.c_double if np.dtype(dtype) == np.float64 else ctypes.c_float
+    fn = lib().ffe_flat_bbg_f64 if np.dtype(dtype) == np.float64 else lib().ffe_flat_bbg_f32
+    fn(ctypes.byref(make_params(sp)), _ptrs(list(out), ct))
+    return out
+
+
+# ---- CFLD files and the reference-build harness ------------------------------
+
+def write_cfld(path: str, U: np.ndarray):
+    U = np.ascontiguousarray(U)
+    nf, nk, nj, ni = U.shape
+    with open(path, "wb") as f:
+        f.write(b"CFLD")
+        np.array([1, U.dtype.itemsize, ni, nj, nk, nf], dtype="<i4").tofile(f)
+        U.astype(U.dtype.newbyteorder("<")).tofile(f)
+
+
+def read_cfld(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        assert f.read(4) == b"CFLD"
+        hdr = np.fromfile(f, dtype="<i4", count=6)
+        _, isz, ni, nj, nk, nf = [int(v) for v in hdr]
+        dt = np.dtype("<f8") if isz == 8 else np.dtype("<f4")
+        data = np.fromfile(f, dtype=dt, count=nf * nk * nj * ni)
+    return data.reshape(nf, nk, nj, ni).astype(dt.newbyteorder("="))
+
+
+def reference_available(dtype=np.float64) -> bool:
+    name = "ref_ez_f64" if np.dtype(dtype) == np.float64 else "ref_ez_f32"
+    return os.path.exists(os.path.join(REF_DIR, name))
+
+
+def run_reference(sp, U0: np.ndarray, nsteps: int, dumps: Sequence[int] = (),
+                  norms: bool = False, timing_warmup: int = -1, env=None):
+    """Run the reference CPU build (oracle/_ref) from U0 for nsteps.
+
+    Returns (dict step -> (7,k,j,i) array, norms array or None, stderr text)."""
+    dtype = U0.dtype
+    exe = os.path.join(REF_DIR, "ref_ez_f64" if dtype == np.float64 else "ref_ez_f32")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"reference harness not built: {exe}")
+    tmp = tempfile.mkdtemp(prefix="coffee_ref_")
+    try:
+        with open(os.path.join(tmp, "config.toml"), "w") as f:
+            f.write(sp.to_toml())
+        write_cfld(os.path.join(tmp, "in.cfld"), U0)
+        args = [exe, "in.cfld", "out", str(nsteps),
+                ",".join(str(s) for s in dumps) if dumps else "-",
+                "norms.txt" if norms else "-", str(timing_warmup)]
+        r = subprocess.run(args, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                           text=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed ({r.returncode}): {r.stderr[-2000:]}")
+        out = {s: read_cfld(os.path.join(tmp, f"out_s{s}.cfld")) for s in dumps}
+        nrm = np.loadtxt(os.path.join(tmp, "norms.txt"), ndmin=2) if norms else None
+        return out, nrm, r.stderr
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.sqrt(np.sum(b * b))
+    return float(np.sqrt(np.sum((a - b) ** 2)) / (den if den > 0 else 1.0))

@@ -1,0 +1,71 @@
+"""Write the golden fixtures of the flat EZ path from the reference's own CPU build.
+
+Runs in the build container only (it needs oracle/_ref, compiled from the
+reference sources by `make -C oracle ref`).  For each case in cases.py and
+each Scalar type it stores, in tests/golden/flat_<case>_<dtype>.npz:
+  - sha256 of the initial state (the tests regenerate it from cases.py and
+    check the hash, so the fixture pins the same input);
+  - for every step 1..nsteps: interior sum of squares of the 7 fields;
+  - at the DUMPS steps: the 7 field values at 512 seeded sample cells
+    (interior and guard cells alike).
+The 128^3 x 100-step wave (flat_c1_norms_f64.npz) stores the per-step norms
+and 512 sampled cells after step 100.
+
+Usage: python tests/golden/make_golden.py
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from tests.golden.cases import CASES, DUMPS, case_wave  # noqa: E402
+
+NSAMPLE = 512
+
+
+def sample_index(shape, seed=12345):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    return np.sort(rng.choice(n, size=min(NSAMPLE, n), replace=False))
+
+
+def digest(U: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(U).tobytes()).hexdigest()
+
+
+def main():
+    for name, fn in CASES.items():
+        for dtype in (np.float64, np.float32):
+            sp, U0 = fn(dtype)
+            nsteps = max(DUMPS)
+            out, nrm, _ = O.run_reference(sp, U0, nsteps, dumps=DUMPS, norms=True)
+            idx = sample_index(U0.shape[1:])
+            rec = {"sha256": np.array(digest(U0)), "norms": nrm, "index": idx,
+                   "dumps": np.array(DUMPS)}
+            for s in DUMPS:
+                rec[f"s{s}"] = out[s].reshape(7, -1)[:, idx]
+            path = os.path.join(HERE, f"flat_{name}_{np.dtype(dtype).name}.npz")
+            np.savez_compressed(path, **rec)
+            print("wrote", path)
+    # C1 at full size: 128^3, FP64, 100 steps
+    sp, U0 = case_wave(np.float64, n=(128, 128, 128), seed=1)
+    out, nrm, _ = O.run_reference(sp, U0, 100, dumps=(100,), norms=True)
+    idx = sample_index(U0.shape[1:])
+    path = os.path.join(HERE, "flat_c1_norms_f64.npz")
+    np.savez_compressed(path, sha256=np.array(digest(U0)), norms=nrm, index=idx,
+                        s100=out[100].reshape(7, -1)[:, idx])
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()
