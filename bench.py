@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the flat EZ field-update hot path on one MI355X.
+
+Workload: the 3D Alfven-wave collision — 512^3 periodic cells (guard 3), FP32,
+background B0 = z, two counter-propagating Gaussian Alfven packets with seeded
+amplitudes, divergence cleaning, E.B / E<B projections and 6th-order
+Kreiss-Oliger on.  A step is one full time step (5 LSRK stages + KO +
+projections + guard fill) of the whole grid, inputs resident in HBM.
+
+Prints one JSON line: cell-updates/s (value), the roofline of the dominant
+kernel (ez_stage: per-launch duration from HIP events recorded on its launch
+stream inside the timed region) and a CPU baseline timed on this host.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+                       [--dtype fp32|fp64] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Algorithmic HBM bytes per cell, in scalars, of the 2N-storage schedule with
+# stencil halos served on chip:
+#   stage 1: read U (7), write U' (7), write dU (7)            = 21
+#   stages 2-4: read U, dU; write U', dU                       = 28 each
+#   stage 5: read U, dU; write U' (dU is dead afterwards)      = 21
+#   Kreiss-Oliger: read U (7), write U' (7)                    = 14
+SCALARS_PER_CELL_UPDATE = 21 + 3 * 28 + 21 + 14  # = 140
+SCALARS_PER_CELL = {"ez_stage": 28, "ez_stage_first": 21, "ez_stage_last": 21, "ez_ko": 14}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--cpu-sample-steps", type=int, default=10)
+    return ap.parse_args()
+
+
+def alfven_params(n: int, dtype):
+    from coffeegpu_amd.params import SimParams
+
+    return SimParams(dt=0.4 / n, N=[n, n, n], guard=[3, 3, 3], lower=[0.0, 0.0, 0.0],
+                     size=[1.0, 1.0, 1.0], nodes=[1, 1, 1], periodic_boundary=[True] * 3,
+                     pml=[0, 0, 0], divB_clean=True, ch2=1.0, tau=2.0, KOeps=0.1,
+                     use_edotb_damping=False, problem=0, calc_current=True, clean_ep=True,
+                     check_egb=True, KO_geometry=False)
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE,
+    copy-kernel calibrated), committed under profiles/ — or None."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def cpu_baseline(args, dtype):
+    """Reference CPU build (oracle/_ref, one MPI rank = one core) on a bounded
+    sample of the same workload; falls back to the oracle port (OpenMP)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from coffeegpu_amd import initial_data as ic
+    from coffeegpu_amd.params import Grid
+
+    n = args.cpu_sample_n
+    sp = alfven_params(n, dtype)
+    g = Grid.from_params(sp, dtype)
+    fields, _ = ic.alfven_packets_seeded(g, args.seed, dtype=dtype)
+    U0 = ic.stack(fields)
+    steps = args.cpu_sample_steps
+    sample = f"{n}^3 cells, {steps} steps after 2 warm-up steps, same physics and dtype"
+    if O.reference_available(dtype):
+        try:
+            _, _, err = O.run_reference(sp, U0, steps + 2, timing_warmup=2)
+            for line in err.splitlines():
+                if line.startswith("TIMING"):
+                    kv = dict(t.split("=") for t in line.split()[1:])
+                    return {"value": float(kv["cell_updates_per_s"]), "unit": "cell-updates/s",
+                            "cores": 1, "kind": "reference",
+                            "sample": sample + "; reference CPU build (use_cuda=0), 1 MPI rank"}
+        except Exception as e:  # reference binary unusable on this host
+            print(f"[bench] reference CPU build unavailable here: {e}", file=sys.stderr)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    st = O.FlatState(U0, dtype)
+    O.flat_evolve(sp, st, 2, mask_lanes=16 if dtype == np.float32 else 8)
+    t0 = time.perf_counter()
+    O.flat_evolve(sp, st, steps, time=2 * sp.dt, mask_lanes=16 if dtype == np.float32 else 8)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port", "sample": sample + f"; project CPU oracle, OpenMP {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not "
+                                   "part of it (the halo exchange is the single-GPU periodic "
+                                   "fill)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from coffeegpu_amd import initial_data as ic
+    from coffeegpu_amd.solver import FieldSolverEZ, SimData, SimEnvironment
+
+    dtype = np.float32 if args.dtype == "fp32" else np.float64
+    n = args.n
+    sp = alfven_params(n, dtype)
+    env = SimEnvironment(sp, dtype, "cuda:0")
+    t0 = time.perf_counter()
+    fields, amps = ic.alfven_packets_seeded(env.grid(), args.seed, dtype=dtype)
+    data = SimData(env)
+    for name, arr in zip(ic.FIELDS, data.state_arrays()):
+        arr.set(fields[name])
+    del fields
+    solver = FieldSolverEZ(data, env)
+    print(f"[bench] {n}^3 {args.dtype} set-up {time.perf_counter() - t0:.1f}s, amplitudes "
+          f"{amps[0]:.3f} {amps[1]:.3f}", file=sys.stderr, flush=True)
+
+    t = 0.0
+    for _ in range(args.warmup):
+        solver.evolve_fields(t)
+        t += sp.dt
+    torch.cuda.synchronize()
+    solver.set_timing(True)
+    solver.reset_timing()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        solver.evolve_fields(t)
+        t += sp.dt
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = start.elapsed_time(stop)
+    timing = solver.get_timing()
+    solver.set_timing(False)
+    res = solver.residuals()
+    ms_per_step = ms_total / args.steps
+    cells = n ** 3
+    value = cells / (ms_per_step * 1e-3)
+
+    esz = np.dtype(dtype).itemsize
+    cnt, ms = timing["ez_stage"]
+    avg_ms = ms / cnt
+    bytes_per_launch = SCALARS_PER_CELL["ez_stage"] * esz * cells
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"ez_stage_{n}_{args.dtype}_traffic.json")
+    kernels = {k: {"launches": c, "avg_ms": m / c,
+                   "algorithmic_GBps": SCALARS_PER_CELL.get(k, 0) * esz * cells / (m / c * 1e-3) / 1e9
+                   if k in SCALARS_PER_CELL else None}
+               for k, (c, m) in timing.items()}
+    step_gbs = SCALARS_PER_CELL_UPDATE * esz * value / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, dtype)
+
+    out = {
+        "metric": "cell-updates/s (flat EZ 3D Alfven collision, 1 GPU)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": "synthetic (seeded Alfven packets)",
+        "config": {"workload": f"alfven3d {n}^3 periodic, guard 3, {args.dtype}, LSRK(5,4)+KO6, "
+                               "cleaning + E.B/E<B projections",
+                   "cells": cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (traffic or {}).get("bytes_per_launch"),
+                     "kernel": "ez_stage",
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "step_algorithmic_GBps": step_gbs,
+        "kernels": kernels,
+        "wall_s_timed": wall,
+        "residuals_after": res,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

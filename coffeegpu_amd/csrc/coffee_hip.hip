@@ -1,0 +1,528 @@
+// C ABI of the gfx950 force-free field-update hot path (include/coffee_hip.h).
+//
+// A context holds the solver variant, the grid, the kernel parameter block in
+// the context's Scalar precision and the scratch it owns: the second buffer of
+// the ping-pong state (7 arrays of the full guarded extent), the reduction
+// buffers and the timing events.  Caller arrays are never allocated or freed
+// here.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "coffee_hip.h"
+#include "ez_flat.hpp"
+
+using namespace coffee;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct TimingRec {
+  int name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct coffee_ctx {
+  int solver = 0;
+  int scalar = 8;
+  int device = 0;
+  coffee_grid grid{};
+  coffee_params params{};
+  FlatK<float> kf{};
+  FlatK<double> kd{};
+  void* scratch[7] = {};  // ping-pong partner buffers
+  double* d_part = nullptr;
+  int part_blocks = 0;
+  std::vector<double> h_part;
+  int diagnostics = 0;
+  int timing = 0;
+  std::vector<std::string> tnames;
+  std::map<std::string, int> tindex;
+  std::vector<TimingRec> trecs;
+  std::vector<hipEvent_t> event_pool;
+  std::string err;
+};
+
+namespace {
+
+int fail(coffee_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(ctx, COFFEE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t cells(const coffee_grid& g) { return (size_t)g.dims[0] * g.dims[1] * g.dims[2]; }
+
+template <typename T>
+void fill_k(const coffee_grid& g, const coffee_params& p, FlatK<T>& k) {
+  for (int a = 0; a < 3; a++) {
+    k.dims[a] = g.dims[a];
+    k.guard[a] = g.guard[a];
+    k.n[a] = g.dims[a] - 2 * g.guard[a];
+    k.per[a] = p.periodic[a] ? 1 : 0;
+    k.delta[a] = (T)g.delta[a];
+    k.lower[a] = (T)g.lower[a];
+    // inv_delta = 1.0 / delta in Scalar (src/sim_env_impl.hpp:57-58)
+    T inv = (T)(1.0 / (double)k.delta[a]);
+    k.c12[a] = inv / T(12);
+    T lo = (T)p.lower[a], sz = (T)p.size[a];
+    k.xh[a] = lo + sz - (T)p.pml[a] * k.delta[a];
+    k.xl[a] = lo + (T)p.pml[a] * k.delta[a];
+    k.pmlscale[a] = (T)p.pmllen * k.delta[a];
+  }
+  k.sy = g.dims[0];
+  k.sz = g.dims[0] * g.dims[1];
+  k.dt = (T)p.dt;
+  k.ch2 = (T)p.ch2;
+  k.tau = (T)p.tau;
+  // -KOeps * KO_const with KO_const = -1/64 (field_solver_EZ.cpp:379-388)
+  k.KOfac = -(T)p.KOeps * (T)(-1.0 / 64.0);
+  k.damp_over_dt = (T)p.damp_gamma / (T)p.dt;
+  k.divB_clean = p.divB_clean;
+  k.use_edotb = p.use_edotb_damping;
+  k.problem = p.problem;
+  k.pert_type = p.pert_type;
+  k.mask_lanes = p.mask_lanes > 0 ? p.mask_lanes : 1;
+  k.pml_any = (p.pml[0] > 0 || p.pml[1] > 0 || p.pml[2] > 0) ? 1 : 0;
+  k.sigpml = (T)p.sigpml;
+  k.radius = (T)p.radius;
+  k.omega = (T)p.omega;
+  k.b0 = (T)p.b0;
+  k.p1 = (T)p.p1; k.p2 = (T)p.p2; k.p3 = (T)p.p3;
+  k.q11 = (T)p.q11; k.q12 = (T)p.q12; k.q13 = (T)p.q13; k.q22 = (T)p.q22; k.q23 = (T)p.q23;
+  k.qox = (T)p.q_offset_x; k.qoy = (T)p.q_offset_y; k.qoz = (T)p.q_offset_z;
+  k.tp_start = (T)p.tp_start; k.tp_end = (T)p.tp_end; k.rpert1 = (T)p.rpert1;
+  k.rpert2 = (T)p.rpert2; k.dw0 = (T)p.dw0; k.nT = (T)p.nT; k.ph1 = (T)p.ph1; k.ph2 = (T)p.ph2;
+  k.dph = (T)p.dph; k.theta0 = (T)p.theta0; k.drpert = (T)p.drpert;
+  k.tp_start1 = (T)p.tp_start1; k.tp_end1 = (T)p.tp_end1; k.rpert11 = (T)p.rpert11;
+  k.rpert21 = (T)p.rpert21; k.dw1 = (T)p.dw1; k.nT1 = (T)p.nT1; k.ph11 = (T)p.ph11;
+  k.ph21 = (T)p.ph21; k.dph1 = (T)p.dph1;
+}
+
+int validate(coffee_ctx* c, const coffee_grid* g, const coffee_params* p) {
+  for (int a = 0; a < 3; a++) {
+    if (g->dims[a] < 1 || g->guard[a] < 0 || g->dims[a] - 2 * g->guard[a] < 1)
+      return fail(c, COFFEE_ERR_INVALID, "invalid grid dims/guard");
+    if (g->guard[a] < 3)
+      return fail(c, COFFEE_ERR_INVALID, "guard must be >= 3 on every axis (KO6 stencil radius)");
+    if (p->periodic[a] && g->dims[a] - 2 * g->guard[a] < 3)
+      return fail(c, COFFEE_ERR_INVALID, "periodic axis needs at least 3 interior cells");
+  }
+  double total = (double)g->dims[0] * g->dims[1] * g->dims[2];
+  if (total >= 2147483647.0) return fail(c, COFFEE_ERR_SIZE, "grid exceeds 2^31-1 cells per array");
+  if (!(p->dt > 0)) return fail(c, COFFEE_ERR_INVALID, "dt must be positive");
+  if (p->tau == 0) return fail(c, COFFEE_ERR_INVALID, "tau must be nonzero");
+  return COFFEE_OK;
+}
+
+hipEvent_t take_event(coffee_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int tname(coffee_ctx* c, const char* n) {
+  auto it = c->tindex.find(n);
+  if (it != c->tindex.end()) return it->second;
+  int id = (int)c->tnames.size();
+  c->tnames.push_back(n);
+  c->tindex[n] = id;
+  return id;
+}
+
+// Launch helper: brackets the launch with events when timing is on.
+template <typename F>
+int timed(coffee_ctx* c, const char* name, hipStream_t s, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (!a || !b) return fail(c, COFFEE_ERR_HIP, "hipEventCreate failed");
+    HIP_TRY(c, hipEventRecord(a, s));
+  }
+  launch();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, COFFEE_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  if (c->timing) {
+    HIP_TRY(c, hipEventRecord(b, s));
+    c->trecs.push_back({tname(c, name), a, b});
+  }
+  return COFFEE_OK;
+}
+
+template <typename T>
+FlatK<T>& kparams(coffee_ctx* c);
+template <>
+FlatK<float>& kparams<float>(coffee_ctx* c) { return c->kf; }
+template <>
+FlatK<double>& kparams<double>(coffee_ctx* c) { return c->kd; }
+
+template <typename T>
+int check_fields(coffee_ctx* c, const coffee_fields* f, bool need_du) {
+  for (int q = 0; q < 3; q++)
+    if (!f->E[q] || !f->B[q]) return fail(c, COFFEE_ERR_INVALID, "E/B pointers must be set");
+  if (!f->P) return fail(c, COFFEE_ERR_INVALID, "P pointer must be set");
+  if (need_du) {
+    for (int q = 0; q < 3; q++)
+      if (!f->dE[q] || !f->dB[q]) return fail(c, COFFEE_ERR_INVALID, "dE/dB pointers must be set");
+    if (!f->dP) return fail(c, COFFEE_ERR_INVALID, "dP pointer must be set");
+  }
+  return COFFEE_OK;
+}
+
+template <typename T>
+void state_ptrs(const coffee_fields* f, T* out[7]) {
+  for (int q = 0; q < 3; q++) {
+    out[q] = (T*)f->E[q];
+    out[3 + q] = (T*)f->B[q];
+  }
+  out[6] = (T*)f->P;
+}
+
+template <typename T>
+int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bool wrap) {
+  FlatK<T>& k = kparams<T>(c);
+  Ptr7<T> d, r;
+  for (int q = 0; q < 7; q++) {
+    d.p[q] = dst[q];
+    r.p[q] = src[q];
+  }
+  for (int ax = 0; ax < 3; ax++) {
+    if (k.guard[ax] == 0) continue;
+    if (wrap != (bool)k.per[ax]) continue;
+    long ext = 2L * k.guard[ax];
+    for (int a = 0; a < 3; a++)
+      if (a != ax) ext *= k.dims[a];
+    int blocks = (int)std::min<long>((ext + 255) / 256, 4096);
+    int rc;
+    if (wrap)
+      rc = timed(c, "guard_fill", s, [&] {
+        guard_slab_kernel<T, true><<<blocks, 256, 0, s>>>(k, ax, d, r, 7);
+      });
+    else
+      rc = timed(c, "guard_copy", s, [&] {
+        guard_slab_kernel<T, false><<<blocks, 256, 0, s>>>(k, ax, d, r, 7);
+      });
+    if (rc) return rc;
+  }
+  return COFFEE_OK;
+}
+
+template <typename T>
+int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s) {
+  static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
+                               -1.514183444257};
+  static const double Bs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
+                               0.6994504559488, 0.1530572479681};
+  static const double cs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345,
+                               0.9582821306784};
+  int rc = check_fields<T>(c, f, true);
+  if (rc) return rc;
+  FlatK<T>& k = kparams<T>(c);
+  if (k.pml_any) {
+    for (int q = 0; q < 3; q++)
+      if (!f->Etmp[q] || !f->Btmp[q] || !f->Bbg[q])
+        return fail(c, COFFEE_ERR_INVALID, "absorbing layer needs Etmp, Btmp and Bbg");
+    if (!f->Ptmp) return fail(c, COFFEE_ERR_INVALID, "absorbing layer needs Ptmp");
+  }
+  if (c->diagnostics) {
+    for (int q = 0; q < 3; q++)
+      if (!f->J[q]) return fail(c, COFFEE_ERR_INVALID, "diagnostics need J");
+    if (!f->divE || !f->divB) return fail(c, COFFEE_ERR_INVALID, "diagnostics need divE/divB");
+  }
+  T* A[7];
+  T* B[7];
+  state_ptrs<T>(f, A);
+  for (int q = 0; q < 7; q++) B[q] = (T*)c->scratch[q];
+
+  // Frozen (non-periodic) guards must be present in the partner buffer.
+  rc = guard_slabs<T>(c, s, B, A, false);
+  if (rc) return rc;
+
+  FlatIO<T> io{};
+  for (int q = 0; q < 3; q++) {
+    io.dU[q] = (T*)f->dE[q];
+    io.dU[3 + q] = (T*)f->dB[q];
+    io.Un[q] = (T*)f->Etmp[q];
+    io.Un[3 + q] = (T*)f->Btmp[q];
+    io.Bbg[q] = (const T*)f->Bbg[q];
+    io.J[q] = (T*)f->J[q];
+  }
+  io.dU[6] = (T*)f->dP;
+  io.Un[6] = (T*)f->Ptmp;
+  io.divE = (T*)f->divE;
+  io.divB = (T*)f->divB;
+
+  dim3 block(64, 4, 1);
+  dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
+  const T dt = k.dt;
+  const T t0 = (T)time;
+  const bool dg = c->diagnostics != 0;
+  for (int st = 0; st < 5; st++) {
+    T** in = (st % 2 == 0) ? A : B;
+    T** out = (st % 2 == 0) ? B : A;
+    for (int q = 0; q < 7; q++) {
+      io.in[q] = in[q];
+      io.out[q] = out[q];
+    }
+    const T as = (T)As[st], bs = (T)Bs[st];
+    const T ts = t0 + (T)cs[st] * dt;
+    if (st == 0)
+      rc = timed(c, "ez_stage_first", s, [&] {
+        if (dg) ez_stage_kernel<T, true, false, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+        else ez_stage_kernel<T, true, false, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+      });
+    else if (st == 4)
+      rc = timed(c, "ez_stage_last", s, [&] {
+        if (dg) ez_stage_kernel<T, false, true, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+        else ez_stage_kernel<T, false, true, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+      });
+    else
+      rc = timed(c, "ez_stage", s, [&] {
+        if (dg) ez_stage_kernel<T, false, false, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+        else ez_stage_kernel<T, false, false, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+      });
+    if (rc) return rc;
+  }
+  // Kreiss-Oliger: B -> A
+  for (int q = 0; q < 7; q++) {
+    io.in[q] = B[q];
+    io.out[q] = A[q];
+  }
+  const T t1 = t0 + dt;
+  rc = timed(c, "ez_ko", s, [&] { ez_ko_kernel<T><<<grid, block, 0, s>>>(k, io, t1); });
+  if (rc) return rc;
+  // single-GPU exchange: periodic guards of the caller's arrays
+  return guard_slabs<T>(c, s, A, A, true);
+}
+
+template <typename T>
+int residuals_impl(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s) {
+  int rc = check_fields<T>(c, f, false);
+  if (rc) return rc;
+  FlatK<T>& k = kparams<T>(c);
+  FlatIO<T> io{};
+  T* A[7];
+  state_ptrs<T>(f, A);
+  for (int q = 0; q < 7; q++) io.in[q] = A[q];
+  rc = timed(c, "residuals", s, [&] {
+    residual_kernel<T><<<c->part_blocks, 256, 0, s>>>(k, io, c->d_part);
+  });
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * NRED * c->part_blocks,
+                            hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  double r[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < c->part_blocks; b++) {
+    const double* p = &c->h_part[b * NRED];
+    r[0] += p[0];
+    r[1] = std::max(r[1], p[1]);
+    r[2] = std::max(r[2], p[2]);
+    r[3] = std::max(r[3], p[3]);
+    r[4] += p[4];
+  }
+  for (int q = 0; q < 5; q++) out[q] = r[q];
+  return COFFEE_OK;
+}
+
+template <typename T>
+int energy_impl(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s) {
+  for (int q = 0; q < 3; q++)
+    if (!comp[q]) return fail(c, COFFEE_ERR_INVALID, "total_energy needs 3 component pointers");
+  FlatK<T>& k = kparams<T>(c);
+  int rc = timed(c, "total_energy", s, [&] {
+    energy_kernel<T><<<c->part_blocks, 256, 0, s>>>(k, (const T*)comp[0], (const T*)comp[1],
+                                                    (const T*)comp[2], c->d_part);
+  });
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * c->part_blocks,
+                            hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  double w = 0;
+  for (int b = 0; b < c->part_blocks; b++) w += c->h_part[b];
+  *out = w;
+  return COFFEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int coffee_abi_version(void) { return COFFEE_ABI_VERSION; }
+
+const char* coffee_last_error(const coffee_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
+                      const coffee_params* params, coffee_ctx** out) {
+  if (!out || !grid || !params) return fail(nullptr, COFFEE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (solver != COFFEE_SOLVER_EZ)
+    return fail(nullptr, COFFEE_ERR_INVALID, "solver variant not available in this build");
+  if (scalar_bytes != COFFEE_F32 && scalar_bytes != COFFEE_F64)
+    return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
+  int rc = validate(nullptr, grid, params);
+  if (rc) return rc;
+  coffee_ctx* c = new coffee_ctx();
+  c->solver = solver;
+  c->scalar = scalar_bytes;
+  c->grid = *grid;
+  c->params = *params;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(nullptr, COFFEE_ERR_HIP, "hipGetDevice failed");
+  }
+  fill_k<float>(*grid, *params, c->kf);
+  fill_k<double>(*grid, *params, c->kd);
+  size_t bytes = cells(*grid) * (size_t)scalar_bytes;
+  for (int q = 0; q < 7; q++) {
+    hipError_t e = hipMalloc(&c->scratch[q], bytes);
+    if (e != hipSuccess) {
+      std::string m = std::string("hipMalloc scratch: ") + hipGetErrorString(e);
+      coffee_ctx_destroy(c);
+      return fail(nullptr, COFFEE_ERR_HIP, m);
+    }
+  }
+  c->part_blocks = 1024;
+  c->h_part.assign((size_t)NRED * c->part_blocks, 0.0);
+  if (hipMalloc(&c->d_part, sizeof(double) * NRED * c->part_blocks) != hipSuccess) {
+    coffee_ctx_destroy(c);
+    return fail(nullptr, COFFEE_ERR_HIP, "hipMalloc reduction buffer failed");
+  }
+  *out = c;
+  return COFFEE_OK;
+}
+
+int coffee_ctx_destroy(coffee_ctx* c) {
+  if (!c) return COFFEE_OK;
+  for (int q = 0; q < 7; q++)
+    if (c->scratch[q]) (void)hipFree(c->scratch[q]);
+  if (c->d_part) (void)hipFree(c->d_part);
+  for (auto& r : c->trecs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  delete c;
+  return COFFEE_OK;
+}
+
+int coffee_set_params(coffee_ctx* c, const coffee_params* p) {
+  if (!c || !p) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  int rc = validate(c, &c->grid, p);
+  if (rc) return rc;
+  c->params = *p;
+  fill_k<float>(c->grid, *p, c->kf);
+  fill_k<double>(c->grid, *p, c->kd);
+  return COFFEE_OK;
+}
+
+int coffee_ez_step(coffee_ctx* c, const coffee_fields* f, double time, void* stream) {
+  if (!c || !f) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  if (c->solver != COFFEE_SOLVER_EZ) return fail(c, COFFEE_ERR_STATE, "not a flat EZ context");
+  hipStream_t s = (hipStream_t)stream;
+  return c->scalar == 4 ? ez_step_impl<float>(c, f, time, s) : ez_step_impl<double>(c, f, time, s);
+}
+
+int coffee_fill_guards(coffee_ctx* c, const coffee_fields* f, void* stream) {
+  if (!c || !f) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (c->scalar == 4) {
+    int rc = check_fields<float>(c, f, false);
+    if (rc) return rc;
+    float* A[7];
+    state_ptrs<float>(f, A);
+    return guard_slabs<float>(c, s, A, A, true);
+  }
+  int rc = check_fields<double>(c, f, false);
+  if (rc) return rc;
+  double* A[7];
+  state_ptrs<double>(f, A);
+  return guard_slabs<double>(c, s, A, A, true);
+}
+
+int coffee_total_energy(coffee_ctx* c, void* const comp[3], double* out, void* stream) {
+  if (!c || !comp || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return c->scalar == 4 ? energy_impl<float>(c, comp, out, s) : energy_impl<double>(c, comp, out, s);
+}
+
+int coffee_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], void* stream) {
+  if (!c || !f || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return c->scalar == 4 ? residuals_impl<float>(c, f, out, s) : residuals_impl<double>(c, f, out, s);
+}
+
+int coffee_set_diagnostics(coffee_ctx* c, int enable) {
+  if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  c->diagnostics = enable ? 1 : 0;
+  return COFFEE_OK;
+}
+
+int coffee_set_timing(coffee_ctx* c, int enable) {
+  if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  c->timing = enable ? 1 : 0;
+  return COFFEE_OK;
+}
+
+int coffee_reset_timing(coffee_ctx* c) {
+  if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  for (auto& r : c->trecs) {
+    c->event_pool.push_back(r.a);
+    c->event_pool.push_back(r.b);
+  }
+  c->trecs.clear();
+  return COFFEE_OK;
+}
+
+int coffee_get_timing(coffee_ctx* c, int max_entries, char (*names)[64], double* total_ms,
+                      int* launches, int* n_entries) {
+  if (!c || !n_entries) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  int n = (int)c->tnames.size();
+  std::vector<double> ms(n, 0.0);
+  std::vector<int> cnt(n, 0);
+  if (!c->trecs.empty()) HIP_TRY(c, hipEventSynchronize(c->trecs.back().b));
+  for (auto& r : c->trecs) {
+    float t = 0;
+    HIP_TRY(c, hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.name] += t;
+    cnt[r.name] += 1;
+  }
+  int m = std::min(n, max_entries);
+  for (int q = 0; q < m; q++) {
+    if (names) {
+      std::strncpy(names[q], c->tnames[q].c_str(), 63);
+      names[q][63] = 0;
+    }
+    if (total_ms) total_ms[q] = ms[q];
+    if (launches) launches[q] = cnt[q];
+  }
+  *n_entries = m;
+  return COFFEE_OK;
+}
+
+}  // extern "C"

@@ -149,6 +149,33 @@ def pulsar_dipole(grid: Grid, p: SimParams, dtype=np.float64):
     return _pack(grid, dtype, Bx=bx, By=by, Bz=bz)
 
 
+def periodic_fill(U: np.ndarray, grid: Grid, periodic) -> np.ndarray:
+    """Fill the guard cells of periodic axes with the interior one period away,
+    x then y then z, as send_guard_cells does on one periodic rank
+    (src/sim_env.cpp:365-438).  U is (..., k, j, i); returns a copy."""
+    U = np.array(U, copy=True)
+    for ax in range(3):
+        if not periodic[ax]:
+            continue
+        g = grid.guard[ax]
+        n = grid.reduced_dim(ax)
+        if g == 0:
+            continue
+        nd = U.ndim
+        dim = nd - 1 - ax  # numpy axis of grid axis `ax`
+        idx_lo = [slice(None)] * nd
+        src_lo = [slice(None)] * nd
+        idx_lo[dim] = slice(0, g)
+        src_lo[dim] = slice(n, n + g)
+        U[tuple(idx_lo)] = U[tuple(src_lo)]
+        idx_hi = [slice(None)] * nd
+        src_hi = [slice(None)] * nd
+        idx_hi[dim] = slice(n + g, n + 2 * g)
+        src_hi[dim] = slice(g, 2 * g)
+        U[tuple(idx_hi)] = U[tuple(src_hi)]
+    return U
+
+
 def stack(fields: Dict[str, np.ndarray]) -> np.ndarray:
     """(7, k, j, i) array in the field order Ex..Bz, P."""
     return np.stack([fields[n] for n in FIELDS])

@@ -1,0 +1,175 @@
+/* coffee_hip.h — C ABI of the MI355X (gfx950) force-free field-update hot path.
+ *
+ * Plain C: pointers, sizes and PODs only.  Every entry point is asynchronous
+ * on the hipStream_t passed as `stream` (a `void*`, NULL = the default
+ * stream) unless stated otherwise, returns 0 on success and a negative code on
+ * failure (never throws, never exits), and records a message retrievable with
+ * coffee_last_error().  Arrays are device pointers owned by the caller, in
+ * Fortran order with guard cells: idx = i + (j + k*dims[1])*dims[0]
+ * (src/data/multi_array_impl.hpp:79-91).  A context owns only its scratch
+ * (the second buffer of the ping-pong state, timing events, reduction
+ * buffers); it never frees caller memory.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef COFFEE_HIP_H
+#define COFFEE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COFFEE_ABI_VERSION 1
+
+/* Status codes. */
+#define COFFEE_OK 0
+#define COFFEE_ERR_INVALID (-1)  /* bad argument or configuration */
+#define COFFEE_ERR_HIP (-2)      /* HIP runtime error (message has the hipError string) */
+#define COFFEE_ERR_SIZE (-3)     /* grid too large for this build's index type */
+#define COFFEE_ERR_STATE (-4)    /* call not valid for this context's solver */
+
+/* Solver variants (the three field_solver classes of the reference). */
+#define COFFEE_SOLVER_EZ 0           /* field_solver_EZ (src/algorithms/field_solver_EZ.h:6-33) */
+#define COFFEE_SOLVER_EZ_SPHERICAL 1 /* field_solver_EZ_spherical (field_solver_EZ_spherical.h:6-32) */
+#define COFFEE_SOLVER_GR_EZ 2        /* field_solver_gr_EZ (field_solver_gr_EZ.h:6-32) */
+
+/* Scalar type of the arrays. */
+#define COFFEE_F32 4
+#define COFFEE_F64 8
+
+/* Grid geometry: Coffee::Grid (src/data/grid.h:11-22), held in double. */
+typedef struct coffee_grid {
+  int dims[3];      /* N + 2*guard per axis (1 for an absent axis) */
+  int guard[3];
+  double delta[3];
+  double lower[3];
+  double sizes[3];
+} coffee_grid;
+
+/* Hot-path subset of Coffee::sim_params (src/sim_params.h:15-124), same
+ * names and meaning; dt and tau as in the reference (tau in units of dt). */
+typedef struct coffee_params {
+  double dt;
+  int periodic[3];
+  int pml[3];
+  int pmllen;
+  double sigpml;
+  int divB_clean;
+  double ch2;
+  double tau;
+  double KOeps;
+  int KO_geometry;
+  int use_edotb_damping;
+  double damp_gamma;
+  int calc_current;
+  int clean_ep;
+  int check_egb;
+  int problem;
+  double radius, omega, b0;
+  double p1, p2, p3;
+  double q11, q12, q13, q22, q23;
+  double q_offset_x, q_offset_y, q_offset_z;
+  double tp_start, tp_end, rpert1, rpert2, dw0, nT, ph1, ph2, dph, theta0, drpert;
+  double tp_start1, tp_end1, rpert11, rpert21, dw1, nT1, ph11, ph21, dph1;
+  int pert_type;
+  double a; /* black-hole spin (GR) */
+  double lower[3];
+  double size[3];
+  /* Flat solver: evaluate the tanh mask of the cleaning term at the
+   * positions the reference CPU build uses (its SIMD lane offset is added to
+   * x, y and z alike, field_solver_EZ.cpp:222-224).  Set to the reference
+   * build's vector width (16 for FP32, 8 for FP64 on AVX-512) for parity with
+   * it; 1 evaluates the mask at the node. */
+  int mask_lanes;
+} coffee_params;
+
+/* Device arrays of one solver call: the reference's sim_data members and
+ * the solver's public scratch members, all caller-owned.  Unused entries
+ * may be NULL (see each entry point).  For the GR solver E/B hold D/B. */
+typedef struct coffee_fields {
+  void *E[3];      /* sim_data::E */
+  void *B[3];      /* sim_data::B */
+  void *P;         /* sim_data::P */
+  void *dE[3];     /* solver dE  (LSRK register) */
+  void *dB[3];     /* solver dB */
+  void *dP;        /* solver dP */
+  void *Etmp[3];   /* solver Etmp: step-start E in the absorbing layer */
+  void *Btmp[3];   /* solver Btmp */
+  void *Ptmp;      /* solver Ptmp */
+  void *Bbg[3];    /* solver Bbg: layer background (flat CPU semantics) */
+  void *J[3];      /* sim_data::B0 (holds J) — written only when diagnostics are on */
+  void *divE;      /* sim_data::divE — diagnostics */
+  void *divB;      /* sim_data::divB — diagnostics */
+} coffee_fields;
+
+typedef struct coffee_ctx coffee_ctx;
+
+/* Create a solver context.  Replaces the constructors
+ * field_solver_EZ(sim_data&, sim_environment&) (field_solver_EZ.cpp:50-101,
+ * field_solver_EZ.cu:709-751) and the constant-memory upload of sim_params /
+ * Grid (src/cuda/constant_mem.cu:7-21, src/cuda/sim_env.cpp:250-268).
+ * Allocates the context's scratch on the current HIP device. */
+int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid *grid,
+                      const coffee_params *params, coffee_ctx **out);
+
+/* Destroy a context and free its scratch (never caller arrays). */
+int coffee_ctx_destroy(coffee_ctx *ctx);
+
+/* Replace the parameters (e.g. dt, omega) between steps: the reference's
+ * init_dev_params (src/cuda/constant_mem_func.h). Grid is fixed. */
+int coffee_set_params(coffee_ctx *ctx, const coffee_params *params);
+
+/* One full time step: 5 LSRK(5,4) stages (stencil, current, cleaning, dU
+ * and U update, E.B / E<B projection, problem boundary, stage-5 absorbing
+ * layer), Kreiss-Oliger, projection, boundary and the single-GPU guard fill.
+ * Replaces field_solver_EZ::evolve_fields(Scalar time)
+ * (field_solver_EZ.cpp:1007-1073, field_solver_EZ.cu:656-707) together with
+ * the sim_environment::send_guard_cells calls it makes
+ * (src/sim_env_impl.hpp:179-195).  On return (stream order) the state is back
+ * in the caller's E, B, P.  Needs E, B, P, dE, dB, dP; Etmp/Btmp/Ptmp and Bbg
+ * when an absorbing layer is active. */
+int coffee_ez_step(coffee_ctx *ctx, const coffee_fields *f, double time, void *stream);
+
+/* Guard-cell fill on one GPU: periodic axes take the interior one period
+ * away (x, then y, then z); other axes are left untouched (frozen), as
+ * send_guard_cells does on a single rank (src/sim_env.cpp:365-438). */
+int coffee_fill_guards(coffee_ctx *ctx, const coffee_fields *f, void *stream);
+
+/* Sum over interior cells of f0^2+f1^2+f2^2, excluding the absorbing band
+ * and (problem 1) r < radius: field_solver_EZ::total_energy
+ * (field_solver_EZ.cu:755-795).  Synchronous on `stream`; result in *out. */
+int coffee_total_energy(coffee_ctx *ctx, void *const comp[3], double *out, void *stream);
+
+/* Residual monitor over interior cells (no reference counterpart; the
+ * reference writes divE/divB/J arrays each stage instead): out[0] = field
+ * energy sum(E^2+B^2), out[1] = max |div B|, out[2] = max |E.B|/max(B^2,TINY),
+ * out[3] = max (E^2-B^2)/max(B^2,TINY) (> 0 means E > B somewhere),
+ * out[4] = number of non-finite cells.  Synchronous on `stream`. */
+int coffee_residuals(coffee_ctx *ctx, const coffee_fields *f, double out[5], void *stream);
+
+/* Diagnostics mode: when on, each stage also writes J (into f->J), divE and
+ * divB exactly as the reference does every stage (field_solver_EZ.cpp:123-125,
+ * 149-150,198-200); off (default) in timed runs. */
+int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
+
+/* Per-kernel timing with HIP events recorded on the launch stream.  When
+ * enabled, every launch is bracketed by events; coffee_get_timing returns,
+ * per kernel name, the number of launches and the summed milliseconds since
+ * the last reset (synchronises the stream of the last launch). */
+int coffee_set_timing(coffee_ctx *ctx, int enable);
+int coffee_get_timing(coffee_ctx *ctx, int max_entries, char (*names)[64], double *total_ms,
+                      int *launches, int *n_entries);
+int coffee_reset_timing(coffee_ctx *ctx);
+
+/* Last error message of this context (or of the last failed create when ctx
+ * is NULL). */
+const char *coffee_last_error(const coffee_ctx *ctx);
+
+/* Library ABI version (COFFEE_ABI_VERSION). */
+int coffee_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COFFEE_HIP_H */

@@ -50,13 +50,15 @@ def case_noise(dtype, n=(16, 8, 8), seed=7, edotb=True):
     U = 0.3 * rng.standard_normal((7,) + g.shape)
     U[5] += 1.0
     U[6] *= 0.1
+    # guard cells consistent with the periodic wrap (as after an exchange)
+    U = ic.periodic_fill(U, g, sp.periodic_boundary)
     return sp, U.astype(dtype)
 
 
 def case_pulsar(dtype, n=(32, 16, 16)):
     """problem 1: rotating multipole star, non-periodic box with an absorbing
     layer (frozen guards, damping toward Bbg)."""
-    sp = SimParams(dt=0.04, N=list(n), guard=[3, 3, 3], lower=[-2.0, -1.0, -1.0],
+    sp = SimParams(dt=0.04, N=list(n), guard=[3, 3, 3], lower=[-2.0625, -1.0625, -1.0625],
                    size=[4.0, 2.0, 2.0], nodes=[1, 1, 1], periodic_boundary=[False] * 3,
                    pml=[4, 3, 3], pmllen=2, sigpml=0.5, divB_clean=True, ch2=1.0, tau=2.0,
                    KOeps=0.2, use_edotb_damping=False, problem=1, radius=0.55, omega=0.3,
@@ -71,7 +73,7 @@ def case_pulsar(dtype, n=(32, 16, 16)):
 def case_alfven_boundary(dtype, n=(32, 16, 16), pert_type=2):
     """problem 2: dipole star with a time-dependent twist (pert_type 2) or
     angular-velocity pulse (pert_type 0)."""
-    sp = SimParams(dt=0.04, N=list(n), guard=[3, 3, 3], lower=[-2.0, -1.0, -1.0],
+    sp = SimParams(dt=0.04, N=list(n), guard=[3, 3, 3], lower=[-2.0625, -1.0625, -1.0625],
                    size=[4.0, 2.0, 2.0], nodes=[1, 1, 1], periodic_boundary=[False] * 3,
                    pml=[0, 0, 0], pmllen=1, sigpml=10.0, divB_clean=True, ch2=1.0, tau=2.0,
                    KOeps=0.1, use_edotb_damping=False, problem=2, radius=0.6, omega=0.0,

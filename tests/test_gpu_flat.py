@@ -1,0 +1,227 @@
+"""GPU parity of the flat EZ path: HIP kernels through the C ABI vs the CPU oracle.
+
+Tolerances:
+  FP64: <= 1e-12 relative L2 (per E, B vector; P relative to the state norm);
+  FP32: <= 1e-5 relative L2 against the FP64 oracle on the same inputs.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU containers
+    pytest.skip("no GPU", allow_module_level=True)
+
+import oracle as O  # noqa: E402
+from coffeegpu_amd.params import Grid  # noqa: E402
+from coffeegpu_amd.solver import FieldSolverEZ, SimData, SimEnvironment  # noqa: E402
+from tests.golden.cases import CASES, case_alfven, case_noise, case_wave  # noqa: E402
+
+
+def vec_errors(u, r, grid):
+    """Relative L2 errors of E (3), B (3) and P over the interior; P's error
+    is relative to the whole state's norm (P itself may be ~0)."""
+    it = (slice(None),) + grid.interior()
+    u = u[it].astype(np.float64)
+    r = r[it].astype(np.float64)
+    tot = np.sqrt((r ** 2).sum())
+
+    def rel(a, b, den):
+        return float(np.sqrt(((a - b) ** 2).sum()) / max(den, 1e-300))
+
+    eE = rel(u[0:3], r[0:3], np.sqrt((r[0:3] ** 2).sum()))
+    eB = rel(u[3:6], r[3:6], np.sqrt((r[3:6] ** 2).sum()))
+    eP = rel(u[6], r[6], tot)
+    return eE, eB, eP
+
+
+def run_gpu(sp, U0, dtype, nsteps, bbg=None, mask_lanes=None, diag=False):
+    env = SimEnvironment(sp, dtype, "cuda:0")
+    data = SimData(env)
+    data.set_state(U0)
+    solver = FieldSolverEZ(data, env, mask_lanes=mask_lanes)
+    if bbg is not None:
+        for n in range(3):
+            solver.Bbg.data(n).set(bbg[n])
+    if diag:
+        solver.set_diagnostics(True)
+    t = np.dtype(dtype).type(0.0)
+    for _ in range(nsteps):
+        solver.evolve_fields(float(t))
+        t = np.dtype(dtype).type(t + np.dtype(dtype).type(sp.dt))
+    torch.cuda.synchronize()
+    return solver, data
+
+
+def run_oracle(sp, U0, dtype, nsteps, bbg=None, mask_lanes=1, diag=None):
+    st = O.FlatState(U0, dtype, bbg)
+    O.flat_evolve(sp, st, nsteps, mask_lanes=mask_lanes, diag=diag)
+    return st
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("nsteps", [1, 10])
+def test_flat_fp64_matches_oracle(name, nsteps):
+    dtype = np.float64
+    sp, U0 = CASES[name](dtype)
+    g = Grid.from_params(sp, dtype)
+    bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
+    solver, data = run_gpu(sp, U0, dtype, nsteps, bbg)
+    ref = run_oracle(sp, U0, dtype, nsteps, bbg, mask_lanes=solver.mask_lanes)
+    got = data.get_state()
+    eE, eB, eP = vec_errors(got, ref.U, g)
+    print(f"{name} fp64 {nsteps} steps: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-12 and eB <= 1e-12 and eP <= 1e-12
+    # guard cells (periodic fill / frozen) match too
+    assert O.rel_l2(got, ref.U) <= 1e-12
+    solver.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_flat_fp32_matches_fp32_oracle(name):
+    """FP32 GPU vs the FP32 oracle (itself pinned to the reference FP32 build)."""
+    dtype = np.float32
+    sp, U0 = CASES[name](dtype)
+    g = Grid.from_params(sp, dtype)
+    bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
+    solver, data = run_gpu(sp, U0, dtype, 10, bbg)
+    ref = run_oracle(sp, U0, dtype, 10, bbg, mask_lanes=solver.mask_lanes)
+    eE, eB, eP = vec_errors(data.get_state(), ref.U, g)
+    print(f"{name} fp32 vs fp32 oracle, 10 steps: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-5 and eB <= 1e-5 and eP <= 1e-5
+    solver.close()
+
+
+# Smooth problems only: near a multipole singularity the reference's TINY
+# (1e-7 in FP32, 1e-11 in FP64; src/data/typedefs.h:6-12) makes the two
+# precisions differ by design, so the cross-precision contract is not defined
+# for the star problems.
+@pytest.mark.parametrize("name", ["wave", "alfven", "noise"])
+def test_flat_fp32_vs_fp64_oracle(name):
+    sp, U64 = CASES[name](np.float64)
+    g = Grid.from_params(sp, np.float64)
+    U32 = U64.astype(np.float32)
+    solver, data = run_gpu(sp, U32, np.float32, 10, None, mask_lanes=8)
+    ref = run_oracle(sp, U32.astype(np.float64), np.float64, 10, None, mask_lanes=8)
+    eE, eB, eP = vec_errors(data.get_state(), ref.U, g)
+    print(f"{name} fp32 vs fp64 oracle, 10 steps: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-5 and eB <= 1e-5 and eP <= 1e-5
+    solver.close()
+
+
+def test_c1_wave_128_100_steps_fp64_and_fp32():
+    """C1: 128^3 periodic wave, 100 steps.  FP64 GPU vs FP64 oracle <= 1e-12;
+    FP32 GPU vs the FP64 oracle <= 1e-5; both checked against the reference
+    build's per-step norms (golden)."""
+    import os
+
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "flat_c1_norms_f64.npz"))
+    sp, U0 = case_wave(np.float64, n=(128, 128, 128), seed=1)
+    g = Grid.from_params(sp, np.float64)
+    solver, data = run_gpu(sp, U0, np.float64, 100)
+    got64 = data.get_state()
+    solver.close()
+    ref = run_oracle(sp, U0, np.float64, 100, mask_lanes=8)
+    eE, eB, eP = vec_errors(got64, ref.U, g)
+    print(f"C1 fp64 100 steps vs oracle: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-12 and eB <= 1e-12 and eP <= 1e-12
+    it = (slice(None),) + g.interior()
+    norms = (got64[it].astype(np.float64) ** 2).reshape(7, -1).sum(axis=1)
+    gold_last = gold["norms"][-1, 1:]
+    # the reference harness sums 2M squares sequentially: allow its rounding
+    for c in (0, 4, 5):  # Ex, By, Bz carry the wave
+        assert abs(norms[c] - gold_last[c]) <= 1e-9 * abs(gold_last[c])
+    # sampled cells of the reference build after 100 steps, to FP64 rounding
+    samp = got64.reshape(7, -1)[:, gold["index"]]
+    assert np.abs(samp - gold["s100"]).max() <= 1e-13
+    solver32, data32 = run_gpu(sp, U0.astype(np.float32), np.float32, 100, mask_lanes=8)
+    eE, eB, eP = vec_errors(data32.get_state(), ref.U, g)
+    print(f"C1 fp32 100 steps vs fp64 oracle: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-5 and eB <= 1e-5 and eP <= 1e-5
+    solver32.close()
+
+
+def test_diagnostics_mode_matches_oracle():
+    """J, divE, divB written per stage as the reference does (last stage's values)."""
+    dtype = np.float64
+    sp, U0 = case_noise(dtype)
+    g = Grid.from_params(sp, dtype)
+    solver, data = run_gpu(sp, U0, dtype, 1, diag=True)
+    shape = U0.shape[1:]
+    diag = {"J": [np.zeros(shape) for _ in range(3)], "divE": np.zeros(shape),
+            "divB": np.zeros(shape)}
+    run_oracle(sp, U0, dtype, 1, mask_lanes=solver.mask_lanes, diag=diag)
+    it = g.interior()
+    for n in range(3):
+        a = data.B0.data(n).sync_to_host()[it]
+        b = diag["J"][n][it]
+        assert O.rel_l2(a, b) <= 1e-12
+    assert O.rel_l2(data.divE.sync_to_host()[it], diag["divE"][it]) <= 1e-12
+    assert O.rel_l2(data.divB.sync_to_host()[it], diag["divB"][it]) <= 1e-12
+    solver.close()
+
+
+def test_deterministic_replay():
+    """The same run twice is bitwise identical on the GPU."""
+    sp, U0 = case_alfven(np.float32)
+    _, d1 = run_gpu(sp, U0, np.float32, 5)
+    _, d2 = run_gpu(sp, U0, np.float32, 5)
+    assert np.array_equal(d1.get_state(), d2.get_state())
+
+
+def test_energy_and_residuals():
+    dtype = np.float64
+    sp, U0 = case_noise(dtype)
+    g = Grid.from_params(sp, dtype)
+    solver, data = run_gpu(sp, U0, dtype, 0)
+    it = (slice(None),) + g.interior()
+    u = U0[it]
+    wB = solver.total_energy(data.B)
+    # total_energy keeps x > xl strictly (field_solver_EZ.cu:780-781): with
+    # pml = 0 the first interior plane of each axis (x == lower) is excluded
+    x, y, z = g.coords()
+    xs, ys, zs = x[0, 0, g.interior()[2]], y[0, g.interior()[1], 0], z[g.interior()[0], 0, 0]
+    keep = ((zs > sp.lower[2])[:, None, None] & (ys > sp.lower[1])[None, :, None]
+            & (xs > sp.lower[0])[None, None, :])
+    want = ((u[3:6] ** 2).sum(0) * keep).sum()
+    assert abs(wB - want) <= 1e-12 * want
+    r = solver.residuals()
+    E2 = (u[0:3] ** 2).sum(0)
+    B2 = (u[3:6] ** 2).sum(0)
+    assert abs(r["energy"] - (E2 + B2).sum()) <= 1e-12 * (E2 + B2).sum()
+    EB = np.abs((u[0:3] * u[3:6]).sum(0)) / np.maximum(B2, 1e-11)
+    assert abs(r["max_EdotB"] - EB.max()) <= 1e-12 * EB.max()
+    assert r["nonfinite"] == 0
+    # one step later E.B is projected away and E <= B everywhere
+    solver.evolve_fields(0.0)
+    r = solver.residuals()
+    assert r["max_EdotB"] < 1e-12 and r["max_E2mB2"] <= 1e-12
+    solver.close()
+
+
+def test_timing_api_counts_launches():
+    sp, U0 = case_wave(np.float32)
+    env = SimEnvironment(sp, np.float32, "cuda:0")
+    data = SimData(env)
+    data.set_state(U0)
+    solver = FieldSolverEZ(data, env)
+    solver.set_timing(True)
+    for s in range(3):
+        solver.evolve_fields(s * sp.dt)
+    t = solver.get_timing()
+    assert t["ez_stage"][0] == 9 and t["ez_stage_first"][0] == 3 and t["ez_ko"][0] == 3
+    assert all(ms > 0 for _, ms in t.values())
+    solver.close()
+
+
+def test_invalid_params_return_error():
+    from coffeegpu_amd._lib import CoffeeError
+
+    sp, U0 = case_wave(np.float64)
+    bad = sp.replace(dt=-1.0)
+    env = SimEnvironment(bad, np.float64, "cuda:0")
+    data = SimData(env)
+    with pytest.raises(CoffeeError, match="dt must be positive"):
+        FieldSolverEZ(data, env)
