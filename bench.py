@@ -49,9 +49,26 @@ def parse():
     ap.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="0 = LDS-tiled (default), 1 = simple")
+    ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0 = auto)")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run only a plain device copy of this many MiB (3 launches), for "
+                         "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}))
 
 
 def alfven_params(n: int, dtype):
@@ -115,6 +132,9 @@ def cpu_baseline(args, dtype):
 
 def main():
     args = parse()
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not "
                                    "part of it (the halo exchange is the single-GPU periodic "
@@ -136,6 +156,7 @@ def main():
         arr.set(fields[name])
     del fields
     solver = FieldSolverEZ(data, env)
+    solver.set_schedule(args.variant, args.zchunk)
     print(f"[bench] {n}^3 {args.dtype} set-up {time.perf_counter() - t0:.1f}s, amplitudes "
           f"{amps[0]:.3f} {amps[1]:.3f}", file=sys.stderr, flush=True)
 

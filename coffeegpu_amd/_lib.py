@@ -75,6 +75,7 @@ SIGNATURES = {
                                         ctypes.c_double * 5, ctypes.c_void_p]),
     "coffee_set_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "coffee_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "coffee_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "coffee_get_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.POINTER(ctypes.c_int)]),

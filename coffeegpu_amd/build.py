@@ -15,7 +15,9 @@ HIP_SOURCES = ["coffee_hip.hip"]
 HOST_SOURCES = []  # C++ host surface, added when present
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
-HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+# -ffp-contract=off: every fused multiply-add is written out as fma() in the
+# source, so each kernel schedule computes bitwise the same result.
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

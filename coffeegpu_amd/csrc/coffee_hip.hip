@@ -17,6 +17,7 @@
 
 #include "coffee_hip.h"
 #include "ez_flat.hpp"
+#include "ez_flat_tiled.hpp"
 
 using namespace coffee;
 
@@ -45,6 +46,8 @@ struct coffee_ctx {
   std::vector<double> h_part;
   int diagnostics = 0;
   int timing = 0;
+  int variant = 0;  // 0 = LDS-tiled z-marching, 1 = one thread per cell
+  int zchunk = 0;   // planes per workgroup for the tiled kernels (0 = automatic)
   std::vector<std::string> tnames;
   std::map<std::string, int> tindex;
   std::vector<TimingRec> trecs;
@@ -123,8 +126,8 @@ int validate(coffee_ctx* c, const coffee_grid* g, const coffee_params* p) {
       return fail(c, COFFEE_ERR_INVALID, "invalid grid dims/guard");
     if (g->guard[a] < 3)
       return fail(c, COFFEE_ERR_INVALID, "guard must be >= 3 on every axis (KO6 stencil radius)");
-    if (p->periodic[a] && g->dims[a] - 2 * g->guard[a] < 3)
-      return fail(c, COFFEE_ERR_INVALID, "periodic axis needs at least 3 interior cells");
+    if (p->periodic[a] && g->dims[a] - 2 * g->guard[a] < g->guard[a])
+      return fail(c, COFFEE_ERR_INVALID, "periodic axis needs at least `guard` interior cells");
   }
   double total = (double)g->dims[0] * g->dims[1] * g->dims[2];
   if (total >= 2147483647.0) return fail(c, COFFEE_ERR_SIZE, "grid exceeds 2^31-1 cells per array");
@@ -231,6 +234,108 @@ int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], 
   return COFFEE_OK;
 }
 
+// Planes per workgroup: enough workgroups to fill 256 CUs several times
+// over, chunks long enough that the 2R warm-up planes stay a small overhead.
+int auto_zchunk(int tiles_per_plane, int nz) {
+  int zc = 64;
+  while (zc > 16 && (long)tiles_per_plane * ((nz + zc - 1) / zc) < 4096) zc /= 2;
+  return zc;
+}
+
+// Rows per tile of the tiled kernels: variant 0 picks the default for the
+// Scalar type (FP32 8 rows = 512 threads, FP64 4 rows = 256 threads, chosen so
+// the double-buffered LDS planes of the KO pass stay within 64 KiB), 2/3/4
+// force 4/8/16 rows; variant 1 is the thread-per-cell schedule.
+template <typename T>
+int tile_rows(int variant) {
+  if (variant == 2) return 4;
+  if (variant == 3) return 8;
+  if (variant == 4) return sizeof(T) == 4 ? 16 : 8;
+  return sizeof(T) == 4 ? 8 : 4;
+}
+
+template <typename T, bool F, bool L, bool D, bool S, int TY>
+void launch_stage(bool tiled, int tblocks, dim3 grid, dim3 block, hipStream_t s, const FlatK<T>& k,
+                  const FlatIO<T>& io, T as, T bs, T ts, const TileMap& tm) {
+  if (tiled)
+    ez_stage_tiled<T, TY, F, L, D, S><<<tblocks, 64 * TY, 0, s>>>(k, io, as, bs, ts, tm);
+  else
+    ez_stage_kernel<T, F, L, D, S><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+}
+
+template <typename T, bool F, bool L, int TY>
+void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim3 block,
+                     hipStream_t s, const FlatK<T>& k, const FlatIO<T>& io, T as, T bs, T ts,
+                     const TileMap& tm) {
+  if (dg) {
+    if (star) launch_stage<T, F, L, true, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, true, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+  } else {
+    if (star) launch_stage<T, F, L, false, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, false, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+  }
+}
+
+template <typename T, int TY>
+int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7], T* const B[7],
+                  double time, hipStream_t s, bool star) {
+  static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
+                               -1.514183444257};
+  static const double Bs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
+                               0.6994504559488, 0.1530572479681};
+  static const double cs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345,
+                               0.9582821306784};
+  dim3 block(64, 4, 1);
+  dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
+  TileMap tm;
+  tm.ntx = (k.n[0] + 63) / 64;
+  tm.nty = (k.n[1] + TY - 1) / TY;
+  tm.zc = c->zchunk > 0 ? c->zchunk : auto_zchunk(tm.ntx * tm.nty, k.n[2]);
+  if (tm.zc > k.n[2]) tm.zc = k.n[2];
+  tm.nzc = (k.n[2] + tm.zc - 1) / tm.zc;
+  const int tblocks = tm.ntx * tm.nty * tm.nzc;
+  const bool tiled = c->variant != 1;
+  const bool dg = c->diagnostics != 0;
+  const T dt = k.dt;
+  const T t0 = (T)time;
+  int rc = COFFEE_OK;
+  for (int st = 0; st < 5; st++) {
+    T* const* in = (st % 2 == 0) ? A : B;
+    T* const* out = (st % 2 == 0) ? B : A;
+    for (int q = 0; q < 7; q++) {
+      io.in[q] = in[q];
+      io.out[q] = out[q];
+    }
+    const T as = (T)As[st], bs = (T)Bs[st];
+    const T ts = t0 + (T)cs[st] * dt;
+    const char* name = st == 0 ? "ez_stage_first" : (st == 4 ? "ez_stage_last" : "ez_stage");
+    rc = timed(c, name, s, [&] {
+      if (st == 0)
+        launch_stage_fl<T, true, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+      else if (st == 4)
+        launch_stage_fl<T, false, true, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+      else
+        launch_stage_fl<T, false, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    });
+    if (rc) return rc;
+  }
+  // Kreiss-Oliger: B -> A
+  for (int q = 0; q < 7; q++) {
+    io.in[q] = B[q];
+    io.out[q] = A[q];
+  }
+  const T t1 = t0 + dt;
+  return timed(c, "ez_ko", s, [&] {
+    if (tiled) {
+      if (star) ez_ko_tiled<T, TY, true><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
+      else ez_ko_tiled<T, TY, false><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
+    } else {
+      if (star) ez_ko_kernel<T, true><<<grid, block, 0, s>>>(k, io, t1);
+      else ez_ko_kernel<T, false><<<grid, block, 0, s>>>(k, io, t1);
+    }
+  });
+}
+
 template <typename T>
 int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s) {
   static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
@@ -276,44 +381,11 @@ int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t
   io.divE = (T*)f->divE;
   io.divB = (T*)f->divB;
 
-  dim3 block(64, 4, 1);
-  dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
-  const T dt = k.dt;
-  const T t0 = (T)time;
-  const bool dg = c->diagnostics != 0;
-  for (int st = 0; st < 5; st++) {
-    T** in = (st % 2 == 0) ? A : B;
-    T** out = (st % 2 == 0) ? B : A;
-    for (int q = 0; q < 7; q++) {
-      io.in[q] = in[q];
-      io.out[q] = out[q];
-    }
-    const T as = (T)As[st], bs = (T)Bs[st];
-    const T ts = t0 + (T)cs[st] * dt;
-    if (st == 0)
-      rc = timed(c, "ez_stage_first", s, [&] {
-        if (dg) ez_stage_kernel<T, true, false, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-        else ez_stage_kernel<T, true, false, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-      });
-    else if (st == 4)
-      rc = timed(c, "ez_stage_last", s, [&] {
-        if (dg) ez_stage_kernel<T, false, true, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-        else ez_stage_kernel<T, false, true, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-      });
-    else
-      rc = timed(c, "ez_stage", s, [&] {
-        if (dg) ez_stage_kernel<T, false, false, true><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-        else ez_stage_kernel<T, false, false, false><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-      });
-    if (rc) return rc;
-  }
-  // Kreiss-Oliger: B -> A
-  for (int q = 0; q < 7; q++) {
-    io.in[q] = B[q];
-    io.out[q] = A[q];
-  }
-  const T t1 = t0 + dt;
-  rc = timed(c, "ez_ko", s, [&] { ez_ko_kernel<T><<<grid, block, 0, s>>>(k, io, t1); });
+  const bool star = k.problem == 1 || k.problem == 2;
+  const int ty = tile_rows<T>(c->variant);
+  if (ty == 4) rc = ez_launch_all<T, 4>(c, k, io, A, B, time, s, star);
+  else if (ty == 16) rc = ez_launch_all<T, 16>(c, k, io, A, B, time, s, star);
+  else rc = ez_launch_all<T, 8>(c, k, io, A, B, time, s, star);
   if (rc) return rc;
   // single-GPU exchange: periodic guards of the caller's arrays
   return guard_slabs<T>(c, s, A, A, true);
@@ -480,6 +552,15 @@ int coffee_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], void*
 int coffee_set_diagnostics(coffee_ctx* c, int enable) {
   if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
   c->diagnostics = enable ? 1 : 0;
+  return COFFEE_OK;
+}
+
+int coffee_set_schedule(coffee_ctx* c, int variant, int zchunk) {
+  if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  if (variant < 0 || variant > 4) return fail(c, COFFEE_ERR_INVALID, "variant must be 0..4");
+  if (zchunk < 0) return fail(c, COFFEE_ERR_INVALID, "zchunk must be >= 0");
+  c->variant = variant;
+  c->zchunk = zchunk;
   return COFFEE_OK;
 }
 

@@ -63,6 +63,13 @@ struct FlatIO {
   T* __restrict__ divB;
 };
 
+// Streaming store: results are not re-read by this launch, so they bypass
+// the caches' normal allocation and leave L2 to the stencil planes.
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ int wrap_idx(int c, int g, int n, int per) {
   if (per) {
     if (c < g) c += n;
@@ -75,14 +82,19 @@ template <typename T>
 __device__ __forceinline__ T d1(const T* __restrict__ f, int c, int m2, int m1, int p1, int p2,
                                 T c12) {
   // 4th-order centred first derivative (finite_diff_simd.h:50-60,95-97).
-  return ((f[c + m2] - f[c + p2]) + T(8) * (f[c + p1] - f[c + m1])) * c12;
+  return fma(T(8), f[c + p1] - f[c + m1], f[c + m2] - f[c + p2]) * c12;
+}
+
+// delta^6 of seven consecutive values m3..p3 with explicit fused multiply-adds.
+template <typename T>
+__device__ __forceinline__ T d6v(T m3, T m2, T m1, T c0, T p1, T p2, T p3) {
+  return fma(T(-20), c0, fma(T(15), m1 + p1, fma(T(-6), m2 + p2, m3 + p3)));
 }
 
 template <typename T>
 __device__ __forceinline__ T d6(const T* __restrict__ f, int c, const int* o) {
   // delta^6 (finite_diff_simd.h:76-93); o[0..5] = offsets of -3,-2,-1,+1,+2,+3.
-  return (f[c + o[0]] + f[c + o[5]]) - T(6) * (f[c + o[1]] + f[c + o[4]]) +
-         T(15) * (f[c + o[2]] + f[c + o[3]]) - T(20) * f[c];
+  return d6v(f[c + o[0]], f[c + o[1]], f[c + o[2]], f[c], f[c + o[3]], f[c + o[4]], f[c + o[5]]);
 }
 
 template <typename T>
@@ -310,50 +322,27 @@ __device__ __forceinline__ bool in_layer(const FlatK<T>& k, T x, T y, T z) {
   return x > k.xh[0] || x < k.xl[0] || y > k.xh[1] || y < k.xl[1] || z > k.xh[2] || z < k.xl[2];
 }
 
-// Stage kernel.  FIRST: As = 0 (no dU read) and the layer cells of U are
-// saved to Un; LAST: dU is not written back (the next step starts with
-// As = 0) and the absorbing layer is applied.
-template <typename T, bool FIRST, bool LAST, bool DIAG>
-__global__ void __launch_bounds__(256)
-    ez_stage_kernel(const FlatK<T> k, const FlatIO<T> io, const T As, const T Bs, const T tstage) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + k.guard[0];
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + k.guard[1];
-  const int kk = blockIdx.z + k.guard[2];
-  if (i >= k.guard[0] + k.n[0] || j >= k.guard[1] + k.n[1]) return;
-  const int c = i + (j + kk * k.dims[1]) * k.dims[0];
-
-  // neighbour offsets relative to c (periodic axes wrap into the interior)
-  int ox[4], oy[4], oz[4];
-  const int dd[4] = {-2, -1, 1, 2};
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    ox[q] = wrap_idx(i + dd[q], k.guard[0], k.n[0], k.per[0]) - i;
-    oy[q] = (wrap_idx(j + dd[q], k.guard[1], k.n[1], k.per[1]) - j) * k.sy;
-    oz[q] = (wrap_idx(kk + dd[q], k.guard[2], k.n[2], k.per[2]) - kk) * k.sz;
-  }
-#define DX(f) d1(f, c, ox[0], ox[1], ox[2], ox[3], k.c12[0])
-#define DY(f) d1(f, c, oy[0], oy[1], oy[2], oy[3], k.c12[1])
-#define DZ(f) d1(f, c, oz[0], oz[1], oz[2], oz[3], k.c12[2])
-  const T* Ex = io.in[0];
-  const T* Ey = io.in[1];
-  const T* Ez = io.in[2];
-  const T* Bx = io.in[3];
-  const T* By = io.in[4];
-  const T* Bz = io.in[5];
-  const T* P = io.in[6];
-
-  T rotBx = DY(Bz) - DZ(By);
-  T rotBy = DZ(Bx) - DX(Bz);
-  T rotBz = DX(By) - DY(Bx);
-  T rotEx = DY(Ez) - DZ(Ey);
-  T rotEy = DZ(Ex) - DX(Ez);
-  T rotEz = DX(Ey) - DY(Ex);
-  T divE = DX(Ex) + DY(Ey) + DZ(Ez);
-  T divB = DX(Bx) + DY(By) + DZ(Bz);
-
-  T ex = Ex[c], ey = Ey[c], ez = Ez[c];
-  T bx = Bx[c], by = By[c], bz = Bz[c];
-  T pc = P[c];
+// Per-cell body of one LSRK stage, shared by the simple and the tiled kernel.
+// d[f][a] = d(field f)/d(axis a) for f in Ex,Ey,Ez,Bx,By,Bz,P (4th order,
+// already scaled by 1/delta); u[] = the cell's E,B,P.  FIRST: As = 0 (no dU
+// read) and the absorbing-layer cells of U are saved to Un; LAST: dU is not
+// written back (the next step starts with As = 0) and the absorbing layer is
+// applied.
+template <typename T, bool FIRST, bool LAST, bool DIAG, bool STAR>
+__device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& io, const int i,
+                                           const int j, const int kk, const int c,
+                                           const T (&d)[7][3], T (&u)[7], const T (&dprev)[7],
+                                           const T As, const T Bs, const T tstage) {
+  // field_solver_EZ.cpp:139-147
+  const T rotBx = d[5][1] - d[4][2];
+  const T rotBy = d[3][2] - d[5][0];
+  const T rotBz = d[4][0] - d[3][1];
+  const T rotEx = d[2][1] - d[1][2];
+  const T rotEy = d[0][2] - d[2][0];
+  const T rotEz = d[1][0] - d[0][1];
+  const T divE = d[0][0] + d[1][1] + d[2][2];
+  const T divB = d[3][0] + d[4][1] + d[5][2];
+  const T ex = u[0], ey = u[1], ez = u[2], bx = u[3], by = u[4], bz = u[5], pc = u[6];
 
   T B2 = bx * bx + by * by + bz * bz;
   B2 = B2 >= TinyOf<T>::v ? B2 : TinyOf<T>::v;
@@ -384,18 +373,15 @@ __global__ void __launch_bounds__(256)
     // reference CPU build evaluates them (lane offset on x, y and z)
     const int lane = (i - k.guard[0]) % k.mask_lanes;
     const T fl = T(lane);
-    T x = fl * k.delta[0] + node_pos(k, 0, i - lane);
-    T y = fl * k.delta[1] + node_pos(k, 1, j);
-    T z = fl * k.delta[2] + node_pos(k, 2, kk);
-    T r = sqrt(x * x + y * y + z * z);
-    T s = T(0.5) * (T(1) - tanh((r - T(3)) * T(2)));
-    Px = DX(P) * s;
-    Py = DY(P) * s;
-    Pz = DZ(P) * s;
+    const T x = fl * k.delta[0] + node_pos(k, 0, i - lane);
+    const T y = fl * k.delta[1] + node_pos(k, 1, j);
+    const T z = fl * k.delta[2] + node_pos(k, 2, kk);
+    const T r = sqrt(x * x + y * y + z * z);
+    const T s = T(0.5) * (T(1) - tanh((r - T(3)) * T(2)));
+    Px = d[6][0] * s;
+    Py = d[6][1] * s;
+    Pz = d[6][2] * s;
   }
-#undef DX
-#undef DY
-#undef DZ
 
   if (DIAG) {
     io.J[0][c] = Jx;
@@ -416,16 +402,15 @@ __global__ void __launch_bounds__(256)
   du[6] = -pc / k.tau - divB * k.ch2 * dt;
   if (!FIRST) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) du[q] += io.dU[q][c] * As;
+    for (int q = 0; q < 7; q++) du[q] = fma(dprev[q], As, du[q]);
   }
   if (!LAST) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) io.dU[q][c] = du[q];
+    for (int q = 0; q < 7; q++) st_stream(&io.dU[q][c], du[q]);
   }
 
-  T u[7] = {ex, ey, ez, bx, by, bz, pc};
   if (FIRST && k.pml_any) {
-    T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
+    const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
     if (in_layer(k, x, y, z)) {
 #pragma unroll
       for (int q = 0; q < 7; q++) io.Un[q][c] = u[q];
@@ -435,39 +420,88 @@ __global__ void __launch_bounds__(256)
   for (int q = 0; q < 7; q++) u[q] = fma(du[q], Bs, u[q]);
 
   project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
-  if (k.problem == 1 || k.problem == 2) {
-    T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
+  if (STAR) {
+    const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
     star_boundary(k, x, y, z, tstage, u[0], u[1], u[2], u[3], u[4], u[5]);
   }
   if (LAST && k.pml_any) {
     // damping_boundary (damping_boundary.cpp:5-83): E toward En, B toward Bbg
-    T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
+    const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
     if (in_layer(k, x, y, z)) {
-      T sig = pml_sigma(x, k.xl[0], k.xh[0], k.pmlscale[0], k.sigpml) +
-              pml_sigma(y, k.xl[1], k.xh[1], k.pmlscale[1], k.sigpml) +
-              pml_sigma(z, k.xl[2], k.xh[2], k.pmlscale[2], k.sigpml);
+      const T sig = pml_sigma(x, k.xl[0], k.xh[0], k.pmlscale[0], k.sigpml) +
+                    pml_sigma(y, k.xl[1], k.xh[1], k.pmlscale[1], k.sigpml) +
+                    pml_sigma(z, k.xl[2], k.xh[2], k.pmlscale[2], k.sigpml);
       if (sig > TinyOf<T>::v) {
-        T e = exp(-sig), f = (T(1) - e) / sig;
+        const T e = exp(-sig), f = (T(1) - e) / sig;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          T un = io.Un[q][c];
+          const T un = io.Un[q][c];
           u[q] = e * un + f * (u[q] - un);
         }
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          T un = io.Un[3 + q][c], bg = io.Bbg[q][c];
+          const T un = io.Un[3 + q][c], bg = io.Bbg[q][c];
           u[3 + q] = bg + e * (un - bg) + f * (u[3 + q] - un);
         }
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
+  for (int q = 0; q < 7; q++) st_stream(&io.out[q][c], u[q]);
 }
 
-// Kreiss_Oliger + clean_epar_check_eGTb + boundary (field_solver_EZ.cpp:
-// 324-457, 1057-1067): E, B get f += (KOeps/64) sum_axes delta^6 f; P is copied.
-template <typename T>
+// Per-cell end of the Kreiss-Oliger pass (field_solver_EZ.cpp:390-457 with
+// the projection and boundary of .cpp:1057-1067): u[0..5] already hold
+// f + (KOeps/64) sum_axes delta^6 f, u[6] = P (untouched on the CPU path).
+template <typename T, bool STAR>
+__device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T>& io, int i,
+                                               int j, int kk, int c, T (&u)[7], T t) {
+  project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
+  if (STAR) {
+    const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
+    star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
+  }
+#pragma unroll
+  for (int q = 0; q < 7; q++) st_stream(&io.out[q][c], u[q]);
+}
+
+// Simple stage kernel: one thread per cell, stencil neighbours read straight
+// from global memory (periodic axes wrap into the interior).  Used for grids
+// the tiled kernel does not take and as the second implementation in tests.
+template <typename T, bool FIRST, bool LAST, bool DIAG, bool STAR>
+__global__ void __launch_bounds__(256)
+    ez_stage_kernel(const FlatK<T> k, const FlatIO<T> io, const T As, const T Bs, const T tstage) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + k.guard[0];
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + k.guard[1];
+  const int kk = blockIdx.z + k.guard[2];
+  if (i >= k.guard[0] + k.n[0] || j >= k.guard[1] + k.n[1]) return;
+  const int c = i + (j + kk * k.dims[1]) * k.dims[0];
+  int ox[4], oy[4], oz[4];
+  const int dd[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    ox[q] = wrap_idx(i + dd[q], k.guard[0], k.n[0], k.per[0]) - i;
+    oy[q] = (wrap_idx(j + dd[q], k.guard[1], k.n[1], k.per[1]) - j) * k.sy;
+    oz[q] = (wrap_idx(kk + dd[q], k.guard[2], k.n[2], k.per[2]) - kk) * k.sz;
+  }
+  T d[7][3];
+  T u[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) {
+    const T* p = io.in[f];
+    d[f][0] = d1(p, c, ox[0], ox[1], ox[2], ox[3], k.c12[0]);
+    d[f][1] = d1(p, c, oy[0], oy[1], oy[2], oy[3], k.c12[1]);
+    d[f][2] = d1(p, c, oz[0], oz[1], oz[2], oz[3], k.c12[2]);
+    u[f] = p[c];
+  }
+  T dp[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) dp[q] = FIRST ? T(0) : io.dU[q][c];
+  stage_cell<T, FIRST, LAST, DIAG, STAR>(k, io, i, j, kk, c, d, u, dp, As, Bs, tstage);
+}
+
+// Simple Kreiss_Oliger kernel (one thread per cell).
+template <typename T, bool STAR>
 __global__ void __launch_bounds__(256) ez_ko_kernel(const FlatK<T> k, const FlatIO<T> io, const T t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + k.guard[0];
   const int j = blockIdx.y * blockDim.y + threadIdx.y + k.guard[1];
@@ -487,16 +521,10 @@ __global__ void __launch_bounds__(256) ez_ko_kernel(const FlatK<T> k, const Flat
   for (int q = 0; q < 6; q++) {
     const T* f = io.in[q];
     T ko = d6(f, c, ox) + d6(f, c, oy) + d6(f, c, oz);
-    u[q] = f[c] + ko * k.KOfac;
+    u[q] = fma(ko, k.KOfac, f[c]);
   }
   u[6] = io.in[6][c];
-  project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
-  if (k.problem == 1 || k.problem == 2) {
-    T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
-    star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
-  }
-#pragma unroll
-  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
+  ko_cell_finish<T, STAR>(k, io, i, j, kk, c, u, t);
 }
 
 // Guard slabs of one axis: periodic -> copy from the interior one period

@@ -270,6 +270,10 @@ class FieldSolverEZ:
         return {"energy": out[0], "max_divB": out[1], "max_EdotB": out[2], "max_E2mB2": out[3],
                 "nonfinite": out[4]}
 
+    def set_schedule(self, variant: int = 0, zchunk: int = 0):
+        """0 = LDS-tiled z-marching kernels (default), 1 = one thread per cell."""
+        _lib.check(self._lib.coffee_set_schedule(self._ctx, int(variant), int(zchunk)), self._ctx)
+
     def set_timing(self, on: bool):
         _lib.check(self._lib.coffee_set_timing(self._ctx, int(on)), self._ctx)
 

