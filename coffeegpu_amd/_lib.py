@@ -80,6 +80,11 @@ SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.POINTER(ctypes.c_int)]),
     "coffee_reset_timing": (ctypes.c_int, [ctypes.c_void_p]),
+    "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_void_p]),
+    "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,
+                                                 ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_void_p]),
 }
 
 _lib = None

@@ -29,6 +29,29 @@ def hipcc() -> str:
     return "hipcc"
 
 
+DRIVERS = [("coffee_drive_f32", "drivers/coffee_drive.cpp", []),
+           ("coffee_drive_f64", "drivers/coffee_drive.cpp", ["-DUSE_DOUBLE"])]
+
+
+def build_drivers(verbose: bool = False):
+    """C++ drivers over the host surface (include/coffee/*.h), linked to the
+    library with an rpath to this directory."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    for name, src, extra in DRIVERS:
+        out = os.path.join(bindir, name)
+        cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include")] + extra + [
+            "-o", out + ".tmp", os.path.join(CSRC, src), "-L" + HERE, "-lcoffee_hip",
+            "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed for {name}")
+        os.replace(out + ".tmp", out)
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     srcs += [os.path.join(CSRC, s) for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -41,6 +64,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for libcoffee_hip.so")
     os.replace(tmp, OUT)
+    build_drivers(verbose)
     return OUT
 
 

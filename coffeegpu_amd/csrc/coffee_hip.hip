@@ -206,7 +206,15 @@ void state_ptrs(const coffee_fields* f, T* out[7]) {
 }
 
 template <typename T>
-int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bool wrap) {
+__global__ void fill_kernel(T* p, unsigned long long n, T v) {
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename T>
+int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bool wrap,
+                int nfields = 7) {
   FlatK<T>& k = kparams<T>(c);
   Ptr7<T> d, r;
   for (int q = 0; q < 7; q++) {
@@ -223,35 +231,35 @@ int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], 
     int rc;
     if (wrap)
       rc = timed(c, "guard_fill", s, [&] {
-        guard_slab_kernel<T, true><<<blocks, 256, 0, s>>>(k, ax, d, r, 7);
+        guard_slab_kernel<T, true><<<blocks, 256, 0, s>>>(k, ax, d, r, nfields);
       });
     else
       rc = timed(c, "guard_copy", s, [&] {
-        guard_slab_kernel<T, false><<<blocks, 256, 0, s>>>(k, ax, d, r, 7);
+        guard_slab_kernel<T, false><<<blocks, 256, 0, s>>>(k, ax, d, r, nfields);
       });
     if (rc) return rc;
   }
   return COFFEE_OK;
 }
 
-// Planes per workgroup: enough workgroups to fill 256 CUs several times
-// over, chunks long enough that the 2R warm-up planes stay a small overhead.
+// Planes per workgroup: about two workgroups per CU (256 CUs) marching long
+// chunks measured best at 512^3; short grids fall back to chunks of >= 8.
 int auto_zchunk(int tiles_per_plane, int nz) {
-  int zc = 64;
-  while (zc > 16 && (long)tiles_per_plane * ((nz + zc - 1) / zc) < 4096) zc /= 2;
-  return zc;
+  int nzc = (512 + tiles_per_plane - 1) / tiles_per_plane;
+  int zc = (nz + nzc - 1) / nzc;
+  return zc < 8 ? 8 : zc;
 }
 
-// Rows per tile of the tiled kernels: variant 0 picks the default for the
-// Scalar type (FP32 8 rows = 512 threads, FP64 4 rows = 256 threads, chosen so
-// the double-buffered LDS planes of the KO pass stay within 64 KiB), 2/3/4
-// force 4/8/16 rows; variant 1 is the thread-per-cell schedule.
+// Rows per tile of the tiled kernels: variant 0 picks the measured best for
+// the Scalar type on MI355X at 512^3 (FP32 16 rows = 1024 threads, FP64 8
+// rows = 512 threads); 2/3/4 force 4/8/16 rows (16 only in FP32); variant 1
+// is the thread-per-cell schedule.
 template <typename T>
 int tile_rows(int variant) {
   if (variant == 2) return 4;
   if (variant == 3) return 8;
   if (variant == 4) return sizeof(T) == 4 ? 16 : 8;
-  return sizeof(T) == 4 ? 8 : 4;
+  return sizeof(T) == 4 ? 16 : 8;
 }
 
 template <typename T, bool F, bool L, bool D, bool S, int TY>
@@ -552,6 +560,59 @@ int coffee_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], void*
 int coffee_set_diagnostics(coffee_ctx* c, int enable) {
   if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
   c->diagnostics = enable ? 1 : 0;
+  return COFFEE_OK;
+}
+
+int coffee_fill(void* p, unsigned long long n, int scalar_bytes, double value, void* stream) {
+  if (!p && n) return fail(nullptr, COFFEE_ERR_INVALID, "null pointer");
+  if (scalar_bytes != 4 && scalar_bytes != 8)
+    return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
+  if (n == 0) return COFFEE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = (int)std::min<unsigned long long>((n + 255) / 256, 8192);
+  if (scalar_bytes == 4)
+    fill_kernel<float><<<blocks, 256, 0, s>>>((float*)p, n, (float)value);
+  else
+    fill_kernel<double><<<blocks, 256, 0, s>>>((double*)p, n, value);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, std::string("fill: ") + hipGetErrorString(e));
+  return COFFEE_OK;
+}
+
+int coffee_fill_guards_arrays(const coffee_grid* grid, const int periodic[3], void* const* arrays,
+                              int narrays, int scalar_bytes, void* stream) {
+  if (!grid || !periodic || (!arrays && narrays > 0))
+    return fail(nullptr, COFFEE_ERR_INVALID, "null argument");
+  if (scalar_bytes != 4 && scalar_bytes != 8)
+    return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
+  coffee_params p{};
+  p.dt = 1;
+  p.tau = 1;
+  for (int a = 0; a < 3; a++) p.periodic[a] = periodic[a];
+  for (int a = 0; a < 3; a++)
+    if (periodic[a] && (grid->guard[a] > grid->dims[a] - 2 * grid->guard[a] ||
+                        grid->dims[a] - 2 * grid->guard[a] < 1))
+      return fail(nullptr, COFFEE_ERR_INVALID, "periodic axis shorter than its guard");
+  coffee_ctx tmp;
+  fill_k<float>(*grid, p, tmp.kf);
+  fill_k<double>(*grid, p, tmp.kd);
+  hipStream_t s = (hipStream_t)stream;
+  for (int base = 0; base < narrays; base += 7) {
+    const int m = std::min(7, narrays - base);
+    int rc;
+    if (scalar_bytes == 4) {
+      float* A[7] = {};
+      for (int q = 0; q < m; q++) A[q] = (float*)arrays[base + q];
+      for (int q = m; q < 7; q++) A[q] = A[0];
+      rc = guard_slabs<float>(&tmp, s, A, A, true, m);
+    } else {
+      double* A[7] = {};
+      for (int q = 0; q < m; q++) A[q] = (double*)arrays[base + q];
+      for (int q = m; q < 7; q++) A[q] = A[0];
+      rc = guard_slabs<double>(&tmp, s, A, A, true, m);
+    }
+    if (rc) return fail(nullptr, rc, tmp.err);
+  }
   return COFFEE_OK;
 }
 

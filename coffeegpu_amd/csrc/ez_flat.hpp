@@ -63,13 +63,6 @@ struct FlatIO {
   T* __restrict__ divB;
 };
 
-// Streaming store: results are not re-read by this launch, so they bypass
-// the caches' normal allocation and leave L2 to the stencil planes.
-template <typename T>
-__device__ __forceinline__ void st_stream(T* p, T v) {
-  __builtin_nontemporal_store(v, p);
-}
-
 __device__ __forceinline__ int wrap_idx(int c, int g, int n, int per) {
   if (per) {
     if (c < g) c += n;
@@ -406,7 +399,7 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
   }
   if (!LAST) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) st_stream(&io.dU[q][c], du[q]);
+    for (int q = 0; q < 7; q++) io.dU[q][c] = du[q];
   }
 
   if (FIRST && k.pml_any) {
@@ -447,7 +440,7 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
     }
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) st_stream(&io.out[q][c], u[q]);
+  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
 }
 
 // Per-cell end of the Kreiss-Oliger pass (field_solver_EZ.cpp:390-457 with
@@ -462,7 +455,7 @@ __device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T
     star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) st_stream(&io.out[q][c], u[q]);
+  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
 }
 
 // Simple stage kernel: one thread per cell, stencil neighbours read straight

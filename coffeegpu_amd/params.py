@@ -170,7 +170,9 @@ def parse_config(path: str) -> SimParams:
         elif isinstance(cur, bool):
             setattr(p, f_.name, bool(v))
         elif f_.name in _INT_KEYS:
-            setattr(p, f_.name, int(v))
+            # cpptoml's get_as<int> yields nothing for a float value: default kept
+            if isinstance(v, int) and not isinstance(v, bool):
+                setattr(p, f_.name, int(v))
         else:
             setattr(p, f_.name, float(v))
     return p

@@ -170,6 +170,17 @@ int coffee_get_timing(coffee_ctx *ctx, int max_entries, char (*names)[64], doubl
                       int *launches, int *n_entries);
 int coffee_reset_timing(coffee_ctx *ctx);
 
+/* Fill n scalars at device pointer p with `value` (asynchronous).  Replaces
+ * multi_array::assign_dev and its assign_single_value kernel
+ * (src/cuda/data/multi_array.cu:12-19,119-128). */
+int coffee_fill(void *p, unsigned long long n, int scalar_bytes, double value, void *stream);
+
+/* Guard fill of `narrays` arrays of one grid without a solver context: the
+ * same operation as coffee_fill_guards, for sim_environment::send_guard_cells
+ * / send_guard_cell_array callers (src/sim_env_impl.hpp:165-195). */
+int coffee_fill_guards_arrays(const coffee_grid *grid, const int periodic[3], void *const *arrays,
+                              int narrays, int scalar_bytes, void *stream);
+
 /* Last error message of this context (or of the last failed create when ctx
  * is NULL). */
 const char *coffee_last_error(const coffee_ctx *ctx);
