@@ -45,7 +45,9 @@ class CoffeeParams(ctypes.Structure):
         ("tp_end1", ctypes.c_double), ("rpert11", ctypes.c_double),
         ("rpert21", ctypes.c_double), ("dw1", ctypes.c_double), ("nT1", ctypes.c_double),
         ("ph11", ctypes.c_double), ("ph21", ctypes.c_double), ("dph1", ctypes.c_double),
-        ("pert_type", ctypes.c_int), ("a", ctypes.c_double), ("lower", _D3), ("size", _D3),
+        ("pert_type", ctypes.c_int), ("a", ctypes.c_double), ("thp1", ctypes.c_double),
+        ("thp2", ctypes.c_double), ("thp11", ctypes.c_double), ("thp21", ctypes.c_double),
+        ("lower", _D3), ("size", _D3),
         ("mask_lanes", ctypes.c_int),
     ]
 
@@ -80,6 +82,20 @@ SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.POINTER(ctypes.c_int)]),
     "coffee_reset_timing": (ctypes.c_int, [ctypes.c_void_p]),
+    "coffee_sph_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "coffee_sph_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "coffee_sph_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "coffee_sph_num_tiles": (ctypes.c_int, []),
+    "coffee_sph_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "coffee_sph_batch_set_tile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "coffee_sph_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
+    "coffee_sph_batch_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "coffee_sph_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
     "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_void_p]),
     "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,

@@ -16,61 +16,20 @@
 #include <vector>
 
 #include "coffee_hip.h"
+#include "ctx.hpp"
 #include "ez_flat.hpp"
 #include "ez_flat_tiled.hpp"
 
 using namespace coffee;
 
-namespace {
-
-thread_local std::string g_create_error;
-
-struct TimingRec {
-  int name;
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct coffee_ctx {
-  int solver = 0;
-  int scalar = 8;
-  int device = 0;
-  coffee_grid grid{};
-  coffee_params params{};
-  FlatK<float> kf{};
-  FlatK<double> kd{};
-  void* scratch[7] = {};  // ping-pong partner buffers
-  double* d_part = nullptr;
-  int part_blocks = 0;
-  std::vector<double> h_part;
-  int diagnostics = 0;
-  int timing = 0;
-  int variant = 0;  // 0 = LDS-tiled z-marching, 1 = one thread per cell
-  int zchunk = 0;   // planes per workgroup for the tiled kernels (0 = automatic)
-  std::vector<std::string> tnames;
-  std::map<std::string, int> tindex;
-  std::vector<TimingRec> trecs;
-  std::vector<hipEvent_t> event_pool;
-  std::string err;
-};
-
-namespace {
-
-int fail(coffee_ctx* c, int code, const std::string& msg) {
-  if (c)
-    c->err = msg;
-  else
-    g_create_error = msg;
-  return code;
+namespace coffee {
+std::string& create_error() {
+  static thread_local std::string e;
+  return e;
 }
+}  // namespace coffee
 
-#define HIP_TRY(ctx, expr)                                                                  \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail(ctx, COFFEE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+namespace {
 
 size_t cells(const coffee_grid& g) { return (size_t)g.dims[0] * g.dims[1] * g.dims[2]; }
 
@@ -133,46 +92,6 @@ int validate(coffee_ctx* c, const coffee_grid* g, const coffee_params* p) {
   if (total >= 2147483647.0) return fail(c, COFFEE_ERR_SIZE, "grid exceeds 2^31-1 cells per array");
   if (!(p->dt > 0)) return fail(c, COFFEE_ERR_INVALID, "dt must be positive");
   if (p->tau == 0) return fail(c, COFFEE_ERR_INVALID, "tau must be nonzero");
-  return COFFEE_OK;
-}
-
-hipEvent_t take_event(coffee_ctx* c) {
-  if (!c->event_pool.empty()) {
-    hipEvent_t e = c->event_pool.back();
-    c->event_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-int tname(coffee_ctx* c, const char* n) {
-  auto it = c->tindex.find(n);
-  if (it != c->tindex.end()) return it->second;
-  int id = (int)c->tnames.size();
-  c->tnames.push_back(n);
-  c->tindex[n] = id;
-  return id;
-}
-
-// Launch helper: brackets the launch with events when timing is on.
-template <typename F>
-int timed(coffee_ctx* c, const char* name, hipStream_t s, F&& launch) {
-  hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
-    a = take_event(c);
-    b = take_event(c);
-    if (!a || !b) return fail(c, COFFEE_ERR_HIP, "hipEventCreate failed");
-    HIP_TRY(c, hipEventRecord(a, s));
-  }
-  launch();
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, COFFEE_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
-  if (c->timing) {
-    HIP_TRY(c, hipEventRecord(b, s));
-    c->trecs.push_back({tname(c, name), a, b});
-  }
   return COFFEE_OK;
 }
 
@@ -242,106 +161,13 @@ int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], 
   return COFFEE_OK;
 }
 
-// Planes per workgroup: about two workgroups per CU (256 CUs) marching long
-// chunks measured best at 512^3; short grids fall back to chunks of >= 8.
+// Planes per workgroup: one workgroup per CU (256 CUs) marching the whole
+// z extent measured best at 512^3 (fewest warm-up planes, no chunk seams);
+// grids with fewer tiles per plane split z so every CU still gets work.
 int auto_zchunk(int tiles_per_plane, int nz) {
-  int nzc = (512 + tiles_per_plane - 1) / tiles_per_plane;
+  int nzc = (256 + tiles_per_plane - 1) / tiles_per_plane;
   int zc = (nz + nzc - 1) / nzc;
   return zc < 8 ? 8 : zc;
-}
-
-// Rows per tile of the tiled kernels: variant 0 picks the measured best for
-// the Scalar type on MI355X at 512^3 (FP32 16 rows = 1024 threads, FP64 8
-// rows = 512 threads); 2/3/4 force 4/8/16 rows (16 only in FP32); variant 1
-// is the thread-per-cell schedule.
-template <typename T>
-int tile_rows(int variant) {
-  if (variant == 2) return 4;
-  if (variant == 3) return 8;
-  if (variant == 4) return sizeof(T) == 4 ? 16 : 8;
-  return sizeof(T) == 4 ? 16 : 8;
-}
-
-template <typename T, bool F, bool L, bool D, bool S, int TY>
-void launch_stage(bool tiled, int tblocks, dim3 grid, dim3 block, hipStream_t s, const FlatK<T>& k,
-                  const FlatIO<T>& io, T as, T bs, T ts, const TileMap& tm) {
-  if (tiled)
-    ez_stage_tiled<T, TY, F, L, D, S><<<tblocks, 64 * TY, 0, s>>>(k, io, as, bs, ts, tm);
-  else
-    ez_stage_kernel<T, F, L, D, S><<<grid, block, 0, s>>>(k, io, as, bs, ts);
-}
-
-template <typename T, bool F, bool L, int TY>
-void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim3 block,
-                     hipStream_t s, const FlatK<T>& k, const FlatIO<T>& io, T as, T bs, T ts,
-                     const TileMap& tm) {
-  if (dg) {
-    if (star) launch_stage<T, F, L, true, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-    else launch_stage<T, F, L, true, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-  } else {
-    if (star) launch_stage<T, F, L, false, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-    else launch_stage<T, F, L, false, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-  }
-}
-
-template <typename T, int TY>
-int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7], T* const B[7],
-                  double time, hipStream_t s, bool star) {
-  static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
-                               -1.514183444257};
-  static const double Bs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
-                               0.6994504559488, 0.1530572479681};
-  static const double cs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345,
-                               0.9582821306784};
-  dim3 block(64, 4, 1);
-  dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
-  TileMap tm;
-  tm.ntx = (k.n[0] + 63) / 64;
-  tm.nty = (k.n[1] + TY - 1) / TY;
-  tm.zc = c->zchunk > 0 ? c->zchunk : auto_zchunk(tm.ntx * tm.nty, k.n[2]);
-  if (tm.zc > k.n[2]) tm.zc = k.n[2];
-  tm.nzc = (k.n[2] + tm.zc - 1) / tm.zc;
-  const int tblocks = tm.ntx * tm.nty * tm.nzc;
-  const bool tiled = c->variant != 1;
-  const bool dg = c->diagnostics != 0;
-  const T dt = k.dt;
-  const T t0 = (T)time;
-  int rc = COFFEE_OK;
-  for (int st = 0; st < 5; st++) {
-    T* const* in = (st % 2 == 0) ? A : B;
-    T* const* out = (st % 2 == 0) ? B : A;
-    for (int q = 0; q < 7; q++) {
-      io.in[q] = in[q];
-      io.out[q] = out[q];
-    }
-    const T as = (T)As[st], bs = (T)Bs[st];
-    const T ts = t0 + (T)cs[st] * dt;
-    const char* name = st == 0 ? "ez_stage_first" : (st == 4 ? "ez_stage_last" : "ez_stage");
-    rc = timed(c, name, s, [&] {
-      if (st == 0)
-        launch_stage_fl<T, true, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-      else if (st == 4)
-        launch_stage_fl<T, false, true, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-      else
-        launch_stage_fl<T, false, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-    });
-    if (rc) return rc;
-  }
-  // Kreiss-Oliger: B -> A
-  for (int q = 0; q < 7; q++) {
-    io.in[q] = B[q];
-    io.out[q] = A[q];
-  }
-  const T t1 = t0 + dt;
-  return timed(c, "ez_ko", s, [&] {
-    if (tiled) {
-      if (star) ez_ko_tiled<T, TY, true><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
-      else ez_ko_tiled<T, TY, false><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
-    } else {
-      if (star) ez_ko_kernel<T, true><<<grid, block, 0, s>>>(k, io, t1);
-      else ez_ko_kernel<T, false><<<grid, block, 0, s>>>(k, io, t1);
-    }
-  });
 }
 
 template <typename T>
@@ -454,18 +280,18 @@ extern "C" {
 int coffee_abi_version(void) { return COFFEE_ABI_VERSION; }
 
 const char* coffee_last_error(const coffee_ctx* ctx) {
-  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+  return ctx ? ctx->err.c_str() : coffee::create_error().c_str();
 }
 
 int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
                       const coffee_params* params, coffee_ctx** out) {
   if (!out || !grid || !params) return fail(nullptr, COFFEE_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (solver != COFFEE_SOLVER_EZ)
+  if (solver != COFFEE_SOLVER_EZ && solver != COFFEE_SOLVER_EZ_SPHERICAL)
     return fail(nullptr, COFFEE_ERR_INVALID, "solver variant not available in this build");
   if (scalar_bytes != COFFEE_F32 && scalar_bytes != COFFEE_F64)
     return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
-  int rc = validate(nullptr, grid, params);
+  int rc = solver == COFFEE_SOLVER_EZ ? validate(nullptr, grid, params) : COFFEE_OK;
   if (rc) return rc;
   coffee_ctx* c = new coffee_ctx();
   c->solver = solver;
@@ -478,6 +304,22 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
   }
   fill_k<float>(*grid, *params, c->kf);
   fill_k<double>(*grid, *params, c->kd);
+  c->part_blocks = 1024;
+  c->h_part.assign((size_t)NRED * c->part_blocks, 0.0);
+  if (hipMalloc(&c->d_part, sizeof(double) * NRED * c->part_blocks) != hipSuccess) {
+    coffee_ctx_destroy(c);
+    return fail(nullptr, COFFEE_ERR_HIP, "hipMalloc reduction buffer failed");
+  }
+  if (solver == COFFEE_SOLVER_EZ_SPHERICAL) {
+    int rs = coffee::sph_create(c);
+    if (rs) {
+      std::string m = c->err;
+      coffee_ctx_destroy(c);
+      return fail(nullptr, rs, m);
+    }
+    *out = c;
+    return COFFEE_OK;
+  }
   size_t bytes = cells(*grid) * (size_t)scalar_bytes;
   for (int q = 0; q < 7; q++) {
     hipError_t e = hipMalloc(&c->scratch[q], bytes);
@@ -486,12 +328,6 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
       coffee_ctx_destroy(c);
       return fail(nullptr, COFFEE_ERR_HIP, m);
     }
-  }
-  c->part_blocks = 1024;
-  c->h_part.assign((size_t)NRED * c->part_blocks, 0.0);
-  if (hipMalloc(&c->d_part, sizeof(double) * NRED * c->part_blocks) != hipSuccess) {
-    coffee_ctx_destroy(c);
-    return fail(nullptr, COFFEE_ERR_HIP, "hipMalloc reduction buffer failed");
   }
   *out = c;
   return COFFEE_OK;
@@ -502,6 +338,7 @@ int coffee_ctx_destroy(coffee_ctx* c) {
   for (int q = 0; q < 7; q++)
     if (c->scratch[q]) (void)hipFree(c->scratch[q]);
   if (c->d_part) (void)hipFree(c->d_part);
+  delete c->state;
   for (auto& r : c->trecs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -513,6 +350,8 @@ int coffee_ctx_destroy(coffee_ctx* c) {
 
 int coffee_set_params(coffee_ctx* c, const coffee_params* p) {
   if (!c || !p) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  if (c->solver != COFFEE_SOLVER_EZ)
+    return fail(c, COFFEE_ERR_STATE, "set_params is available for the flat solver");
   int rc = validate(c, &c->grid, p);
   if (rc) return rc;
   c->params = *p;
@@ -523,8 +362,9 @@ int coffee_set_params(coffee_ctx* c, const coffee_params* p) {
 
 int coffee_ez_step(coffee_ctx* c, const coffee_fields* f, double time, void* stream) {
   if (!c || !f) return fail(c, COFFEE_ERR_INVALID, "null argument");
-  if (c->solver != COFFEE_SOLVER_EZ) return fail(c, COFFEE_ERR_STATE, "not a flat EZ context");
   hipStream_t s = (hipStream_t)stream;
+  if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) return coffee::sph_step(c, f, time, s);
+  if (c->solver != COFFEE_SOLVER_EZ) return fail(c, COFFEE_ERR_STATE, "unsupported solver");
   return c->scalar == 4 ? ez_step_impl<float>(c, f, time, s) : ez_step_impl<double>(c, f, time, s);
 }
 
@@ -548,6 +388,7 @@ int coffee_fill_guards(coffee_ctx* c, const coffee_fields* f, void* stream) {
 int coffee_total_energy(coffee_ctx* c, void* const comp[3], double* out, void* stream) {
   if (!c || !comp || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
+  if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) return coffee::sph_energy(c, comp, out, s);
   return c->scalar == 4 ? energy_impl<float>(c, comp, out, s) : energy_impl<double>(c, comp, out, s);
 }
 

@@ -1,0 +1,117 @@
+// Context of the C ABI and the launch helpers shared by the solver
+// translation units (coffee_hip.hip: flat EZ and common entry points;
+// coffee_sph.hip: log-spherical and batched runs; coffee_gr.hip: Kerr-Schild).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "coffee_hip.h"
+#include "ez_flat.hpp"
+
+namespace coffee {
+
+struct TimingRec {
+  int name;
+  hipEvent_t a, b;
+};
+
+// Solver-specific state held by a context (spherical batch, GR metric, ...).
+struct SolverState {
+  virtual ~SolverState() {}
+};
+
+std::string& create_error();
+
+}  // namespace coffee
+
+struct coffee_ctx {
+  int solver = 0;
+  int scalar = 8;
+  int device = 0;
+  coffee_grid grid{};
+  coffee_params params{};
+  coffee::FlatK<float> kf{};
+  coffee::FlatK<double> kd{};
+  void* scratch[7] = {};  // ping-pong partner buffers
+  double* d_part = nullptr;
+  int part_blocks = 0;
+  std::vector<double> h_part;
+  int diagnostics = 0;
+  int timing = 0;
+  int variant = 0;  // kernel schedule
+  int zchunk = 0;   // planes per workgroup for the tiled kernels (0 = automatic)
+  std::vector<std::string> tnames;
+  std::map<std::string, int> tindex;
+  std::vector<coffee::TimingRec> trecs;
+  std::vector<hipEvent_t> event_pool;
+  coffee::SolverState* state = nullptr;
+  std::string err;
+};
+
+namespace coffee {
+
+inline int fail(coffee_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    create_error() = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ::coffee::fail(ctx, COFFEE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline hipEvent_t take_event(coffee_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+inline int tname(coffee_ctx* c, const char* n) {
+  auto it = c->tindex.find(n);
+  if (it != c->tindex.end()) return it->second;
+  int id = (int)c->tnames.size();
+  c->tnames.push_back(n);
+  c->tindex[n] = id;
+  return id;
+}
+
+// Launch helper: brackets the launch with events when timing is on.
+template <typename F>
+int timed(coffee_ctx* c, const char* name, hipStream_t s, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (!a || !b) return fail(c, COFFEE_ERR_HIP, "hipEventCreate failed");
+    HIP_TRY(c, hipEventRecord(a, s));
+  }
+  launch();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, COFFEE_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  if (c->timing) {
+    HIP_TRY(c, hipEventRecord(b, s));
+    c->trecs.push_back({tname(c, name), a, b});
+  }
+  return COFFEE_OK;
+}
+
+// Entry points of the other translation units, dispatched from the ABI.
+int sph_create(coffee_ctx* c);
+int sph_step(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s);
+int sph_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s);
+
+}  // namespace coffee

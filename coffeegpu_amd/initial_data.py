@@ -149,6 +149,22 @@ def pulsar_dipole(grid: Grid, p: SimParams, dtype=np.float64):
     return _pack(grid, dtype, Bx=bx, By=by, Bz=bz)
 
 
+def pulsar_dipole_sph(grid: Grid, p: SimParams, dtype=np.float64):
+    """Aligned dipole on the log-spherical grid (x = ln r, y = theta), as
+    user_pulsar_sph.hpp sets it: upper components
+    B^x = b0 2 cos(th) / r^3 / sqrt(g11), B^y = b0 sin(th) / r^3 / sqrt(g22),
+    B^z = 0; E = 0, P = 0 (dipole_sph_2d, pulsar_impl.hpp:131-139)."""
+    tiny = 1e-7 if np.dtype(dtype) == np.float32 else 1e-11
+    x, y, _ = _coords64(grid)
+    r = np.exp(x)
+    g11 = np.maximum(np.sqrt(r * r), tiny)
+    g22 = np.maximum(np.sqrt(r * r), tiny)
+    bx = p.b0 * 2.0 * np.cos(y) / r ** 3 / g11
+    by = p.b0 * np.sin(y) / r ** 3 / g22
+    shape = grid.shape
+    return _pack(grid, dtype, Bx=np.broadcast_to(bx, shape), By=np.broadcast_to(by, shape))
+
+
 def periodic_fill(U: np.ndarray, grid: Grid, periodic) -> np.ndarray:
     """Fill the guard cells of periodic axes with the interior one period away,
     x then y then z, as send_guard_cells does on one periodic rank

@@ -73,6 +73,7 @@ typedef struct coffee_params {
   double tp_start1, tp_end1, rpert11, rpert21, dw1, nT1, ph11, ph21, dph1;
   int pert_type;
   double a; /* black-hole spin (GR) */
+  double thp1, thp2, thp11, thp21; /* spherical twist pulses (wpert_sph_angle) */
   double lower[3];
   double size[3];
   /* Flat solver: evaluate the tanh mask of the cleaning term at the
@@ -169,6 +170,36 @@ int coffee_set_timing(coffee_ctx *ctx, int enable);
 int coffee_get_timing(coffee_ctx *ctx, int max_entries, char (*names)[64], double *total_ms,
                       int *launches, int *n_entries);
 int coffee_reset_timing(coffee_ctx *ctx);
+
+/* ---- batched log-spherical runs (no reference counterpart) --------------
+ * Many independent axisymmetric runs (each its own grid, parameters and
+ * caller-owned E, B, P) advance together: one launch per substep covers a
+ * work list of (run, tile) items; each run keeps its time and step counter on
+ * the device and stops advancing after nsteps[r] steps (0 = never stops).
+ * dE/dB/dP and the absorbing-layer copies are allocated by the batch when
+ * the corresponding fields[r] pointers are NULL.  Results of a run in a
+ * batch are bitwise those of the same run in its own context. */
+typedef struct coffee_ctx coffee_batch;
+int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid *grids,
+                            const coffee_params *params, const coffee_fields *fields,
+                            const int *nsteps, const double *t0, coffee_batch **out);
+int coffee_sph_batch_destroy(coffee_batch *b);
+/* Advance every active run by `nsteps` steps (asynchronous on `stream`). */
+int coffee_sph_batch_step(coffee_batch *b, int nsteps, void *stream);
+/* Tile shapes the kernels are built for: coffee_sph_num_tiles() of them,
+ * tile i being tx x ty cells (threads) per work item. */
+int coffee_sph_num_tiles(void);
+int coffee_sph_tile_shape(int tile_index, int *tx, int *ty);
+int coffee_sph_batch_set_tile(coffee_batch *b, int tile_index);
+/* Time every tile shape on the live batch (first-stage kernel, `trials`
+ * launches each; state is not advanced), keep the fastest (ties within 3%
+ * go to the lower index); ms_per_tile may be NULL. */
+int coffee_sph_batch_autotune(coffee_batch *b, int trials, void *stream, int *best_tile,
+                              double *ms_per_tile);
+/* Capture one step's launch sequence into a HIP graph and replay it. */
+int coffee_sph_batch_use_graph(coffee_batch *b, int enable);
+/* Per-run active flag, time and step count (synchronises the device). */
+int coffee_sph_batch_status(coffee_batch *b, int *active, double *t, int *step);
 
 /* Fill n scalars at device pointer p with `value` (asynchronous).  Replaces
  * multi_array::assign_dev and its assign_single_value kernel

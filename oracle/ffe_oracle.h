@@ -60,6 +60,7 @@ typedef struct ffe_params {
   double tp_start1, tp_end1, rpert11, rpert21, dw1, nT1, ph11, ph21, dph1;
   int pert_type;
   double a;
+  double thp1, thp2, thp11, thp21;
   /* SIMD width of the reference CPU build whose tanh-mask positions are
    * reproduced (field_solver_EZ.cpp:222-224 adds the lane offset to x, y and
    * z alike); 1 evaluates the mask at the node itself. */
@@ -109,6 +110,22 @@ int ffe_flat_boundary_f64(const ffe_params *p, double *const U[7], double t);
  * nodes including guards (field_solver_EZ.cpp:66-85). */
 int ffe_flat_bbg_f64(const ffe_params *p, double *const Bbg[3]);
 int ffe_flat_bbg_f32(const ffe_params *p, float *const Bbg[3]);
+
+/* ---- log-spherical 2D EZ (field_solver_EZ_spherical.cu) ------------------ */
+
+/* Arrays are dims0 x dims1 (x = ln r fastest, then theta); p->guard[1] must
+ * be >= 4.  L: 6 scratch arrays for the lowered El, Bl.  Un: step-start copy
+ * and KO scratch. */
+int ffe_sph_evolve_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                       double *const Un[7], double *const L[6], double time, int nsteps);
+int ffe_sph_evolve_f32(const ffe_params *p, float *const U[7], float *const dU[7],
+                       float *const Un[7], float *const L[6], double time, int nsteps);
+double ffe_sph_energy_f64(const ffe_params *p, double *const F[3]);
+double ffe_sph_energy_f32(const ffe_params *p, float *const F[3]);
+int ffe_sph_axis_f64(const ffe_params *p, double *const U[7]);
+int ffe_sph_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7]);
+int ffe_sph_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                         double *const L[6], double As, double Bs);
 
 /* Closed-form helpers (src/algorithms/pulsar_impl.hpp) in double. */
 double ffe_dipole2(double x, double y, double z, double p1, double p2, double p3,

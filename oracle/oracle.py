@@ -46,7 +46,9 @@ class FFEParams(ctypes.Structure):
         ("rpert11", ctypes.c_double), ("rpert21", ctypes.c_double), ("dw1", ctypes.c_double),
         ("nT1", ctypes.c_double), ("ph11", ctypes.c_double), ("ph21", ctypes.c_double),
         ("dph1", ctypes.c_double),
-        ("pert_type", ctypes.c_int), ("a", ctypes.c_double), ("mask_lanes", ctypes.c_int),
+        ("pert_type", ctypes.c_int), ("a", ctypes.c_double),
+        ("thp1", ctypes.c_double), ("thp2", ctypes.c_double), ("thp11", ctypes.c_double),
+        ("thp21", ctypes.c_double), ("mask_lanes", ctypes.c_int),
     ]
 
 
@@ -208,3 +210,38 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     b = np.asarray(b, dtype=np.float64)
     den = np.sqrt(np.sum(b * b))
     return float(np.sqrt(np.sum((a - b) ** 2)) / (den if den > 0 else 1.0))
+
+
+# ---- log-spherical oracle ------------------------------------------------------
+
+class SphState:
+    """Host state of the spherical solver: U, dU, Un (7 each), L (6)."""
+
+    def __init__(self, U: np.ndarray, dtype):
+        self.dtype = np.dtype(dtype)
+        self.U = np.ascontiguousarray(U, dtype=self.dtype).copy()
+        self.dU = np.zeros_like(self.U)
+        self.Un = np.zeros_like(self.U)
+        self.L = np.zeros((6,) + self.U.shape[1:], dtype=self.dtype)
+
+
+def sph_evolve(sp, state: SphState, nsteps: int, time: float = 0.0):
+    L = lib()
+    p = make_params(sp)
+    ct = ctypes.c_double if state.dtype == np.float64 else ctypes.c_float
+    fn = L.ffe_sph_evolve_f64 if state.dtype == np.float64 else L.ffe_sph_evolve_f32
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.byref(p), _ptrs(list(state.U), ct), _ptrs(list(state.dU), ct),
+            _ptrs(list(state.Un), ct), _ptrs(list(state.L), ct), ctypes.c_double(time),
+            ctypes.c_int(nsteps))
+    if rc != 0:
+        raise RuntimeError(f"oracle sph_evolve failed rc={rc}")
+    return state
+
+
+def sph_energy(sp, F: np.ndarray) -> float:
+    L = lib()
+    ct = ctypes.c_double if F.dtype == np.float64 else ctypes.c_float
+    fn = L.ffe_sph_energy_f64 if F.dtype == np.float64 else L.ffe_sph_energy_f32
+    fn.restype = ctypes.c_double
+    return fn(ctypes.byref(make_params(sp)), _ptrs(list(np.ascontiguousarray(F)), ct))

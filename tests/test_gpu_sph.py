@@ -1,0 +1,155 @@
+"""GPU parity of the log-spherical path and the batched runner.
+
+FP64 GPU vs the FP64 oracle <= 1e-12 (relative L2 of the E and B vectors;
+P relative to the state); FP32 GPU vs the FP32 oracle <= 1e-5.  Batched runs
+equal the same runs done singly, bitwise; so do graph replays and every tile
+shape.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+import oracle as O  # noqa: E402
+from coffeegpu_amd.params import Grid  # noqa: E402
+from coffeegpu_amd.solver import SimData, SimEnvironment  # noqa: E402
+from coffeegpu_amd.spherical import FieldSolverEZSpherical, SphBatch, tile_shapes  # noqa: E402
+from tests.golden.cases_sph import SPH_CASES, case_sph  # noqa: E402
+
+
+def vec_errors(u, r, grid):
+    it = (slice(None),) + grid.interior()
+    u = u[it].astype(np.float64)
+    r = r[it].astype(np.float64)
+    tot = np.sqrt((r ** 2).sum())
+
+    def rel(a, b, den):
+        return float(np.sqrt(((a - b) ** 2).sum()) / max(den, 1e-300))
+
+    return (rel(u[0:3], r[0:3], max(np.sqrt((r[0:3] ** 2).sum()), 1e-30 * tot)),
+            rel(u[3:6], r[3:6], np.sqrt((r[3:6] ** 2).sum())), rel(u[6], r[6], tot))
+
+
+def run_single(sp, U0, dtype, nsteps):
+    env = SimEnvironment(sp, dtype, "cuda:0")
+    data = SimData(env)
+    data.set_state(U0)
+    solver = FieldSolverEZSpherical(data, env)
+    t = np.dtype(dtype).type(0.0)
+    for _ in range(nsteps):
+        solver.evolve_fields(float(t))
+        t = np.dtype(dtype).type(t + np.dtype(dtype).type(sp.dt))
+    torch.cuda.synchronize()
+    out = data.get_state()
+    solver.close()
+    return out
+
+
+@pytest.mark.parametrize("name", list(SPH_CASES))
+@pytest.mark.parametrize("nsteps", [1, 10])
+def test_sph_fp64_matches_oracle(name, nsteps):
+    sp, U0 = case_sph(np.float64, **SPH_CASES[name])
+    g = Grid.from_params(sp, np.float64)
+    got = run_single(sp, U0, np.float64, nsteps)
+    st = O.SphState(U0, np.float64)
+    O.sph_evolve(sp, st, nsteps)
+    eE, eB, eP = vec_errors(got, st.U, g)
+    print(f"sph {name} fp64 {nsteps} steps: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-12 and eB <= 1e-12 and eP <= 1e-12
+    # guard rows (axis mirror) and frozen radial guards match as well
+    assert O.rel_l2(got, st.U) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(SPH_CASES))
+def test_sph_fp32_matches_fp32_oracle(name):
+    sp, U0 = case_sph(np.float32, **SPH_CASES[name])
+    g = Grid.from_params(sp, np.float32)
+    got = run_single(sp, U0, np.float32, 10)
+    st = O.SphState(U0, np.float32)
+    O.sph_evolve(sp, st, 10)
+    eE, eB, eP = vec_errors(got, st.U, g)
+    print(f"sph {name} fp32 10 steps: E {eE:.2e} B {eB:.2e} P {eP:.2e}")
+    assert eE <= 1e-5 and eB <= 1e-5 and eP <= 1e-5
+
+
+def test_sph_energy_matches_oracle():
+    sp, U0 = case_sph(np.float64, omega=0.0, pml=4)
+    env = SimEnvironment(sp, np.float64, "cuda:0")
+    data = SimData(env)
+    data.set_state(U0)
+    solver = FieldSolverEZSpherical(data, env)
+    w = solver.total_energy(data.B)
+    want = O.sph_energy(sp, np.ascontiguousarray(U0[3:6]))
+    assert abs(w - want) <= 1e-12 * abs(want)
+    solver.close()
+
+
+def _batch_setup(dtype, specs):
+    envs, datas, U0s, sps = [], [], [], []
+    for kw in specs:
+        sp, U0 = case_sph(dtype, **kw)
+        env = SimEnvironment(sp, dtype, "cuda:0")
+        d = SimData(env)
+        d.set_state(U0)
+        envs.append(env)
+        datas.append(d)
+        U0s.append(U0)
+        sps.append(sp)
+    return envs, datas, U0s, sps
+
+
+SPECS = [dict(n=(64, 32), omega=0.3), dict(n=(96, 40), omega=0.15, dw0=0.4, edotb=True),
+         dict(n=(48, 24), omega=0.25, pml=0, KO_geometry=False)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_equals_single_runs_bitwise(dtype):
+    envs, datas, U0s, sps = _batch_setup(dtype, SPECS)
+    b = SphBatch(envs, datas, nsteps=[0, 0, 0])
+    b.step(4)
+    torch.cuda.synchronize()
+    for d, U0, sp in zip(datas, U0s, sps):
+        single = run_single(sp, U0, dtype, 4)
+        assert np.array_equal(d.get_state(), single)
+    b.close()
+
+
+def test_batch_tiles_graph_and_retirement():
+    dtype = np.float32
+    ref = None
+    shapes = tile_shapes()
+    assert len(shapes) >= 3
+    for tile in range(len(shapes)):
+        envs, datas, U0s, sps = _batch_setup(dtype, SPECS)
+        b = SphBatch(envs, datas, nsteps=[2, 5, 3])
+        b.set_tile(tile)
+        b.use_graph(tile % 2 == 1)
+        b.step(6)
+        act, t, st = b.status()
+        assert act == [0, 0, 0] and st == [2, 5, 3]
+        outs = [d.get_state() for d in datas]
+        if ref is None:
+            ref = outs
+            for d, U0, sp, n in zip(datas, U0s, sps, (2, 5, 3)):
+                assert np.array_equal(d.get_state(), run_single(sp, U0, dtype, n))
+        else:
+            for a, r in zip(outs, ref):
+                assert np.array_equal(a, r)
+        b.close()
+
+
+def test_autotuner_picks_a_tile_and_keeps_results():
+    dtype = np.float32
+    envs, datas, U0s, sps = _batch_setup(dtype, SPECS)
+    b = SphBatch(envs, datas, nsteps=[0, 0, 0])
+    best, ms = b.autotune(trials=3)
+    assert 0 <= best < len(tile_shapes()) and all(m > 0 for m in ms)
+    b.step(3)
+    for d, U0, sp in zip(datas, U0s, sps):
+        assert np.array_equal(d.get_state(), run_single(sp, U0, dtype, 3))
+    b.close()
