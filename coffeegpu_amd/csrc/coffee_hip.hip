@@ -170,6 +170,100 @@ int auto_zchunk(int tiles_per_plane, int nz) {
   return zc < 8 ? 8 : zc;
 }
 
+// Rows per tile of the tiled kernels: variant 0 picks the measured best for
+// the Scalar type on MI355X at 512^3 (FP32 16 rows = 1024 threads, FP64 8
+// rows = 512 threads); 2/3/4 force 4/8/16 rows (16 only in FP32); variant 1
+// is the thread-per-cell schedule.
+template <typename T>
+int tile_rows(int variant) {
+  if (variant == 2) return 4;
+  if (variant == 3) return 8;
+  if (variant == 4) return sizeof(T) == 4 ? 16 : 8;
+  return sizeof(T) == 4 ? 16 : 8;
+}
+
+template <typename T, bool F, bool L, bool D, bool S, int TY>
+void launch_stage(bool tiled, int tblocks, dim3 grid, dim3 block, hipStream_t s, const FlatK<T>& k,
+                  const FlatIO<T>& io, T as, T bs, T ts, const TileMap& tm) {
+  if (tiled)
+    ez_stage_tiled<T, TY, F, L, D, S><<<tblocks, 64 * TY, 0, s>>>(k, io, as, bs, ts, tm);
+  else
+    ez_stage_kernel<T, F, L, D, S><<<grid, block, 0, s>>>(k, io, as, bs, ts);
+}
+
+template <typename T, bool F, bool L, int TY>
+void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim3 block,
+                     hipStream_t s, const FlatK<T>& k, const FlatIO<T>& io, T as, T bs, T ts,
+                     const TileMap& tm) {
+  if (dg) {
+    if (star) launch_stage<T, F, L, true, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, true, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+  } else {
+    if (star) launch_stage<T, F, L, false, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, false, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+  }
+}
+
+template <typename T, int TY>
+int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7], T* const B[7],
+                  double time, hipStream_t s, bool star) {
+  static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
+                               -1.514183444257};
+  static const double Bs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
+                               0.6994504559488, 0.1530572479681};
+  static const double cs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345,
+                               0.9582821306784};
+  dim3 block(64, 4, 1);
+  dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
+  TileMap tm;
+  tm.ntx = (k.n[0] + 63) / 64;
+  tm.nty = (k.n[1] + TY - 1) / TY;
+  tm.zc = c->zchunk > 0 ? c->zchunk : auto_zchunk(tm.ntx * tm.nty, k.n[2]);
+  if (tm.zc > k.n[2]) tm.zc = k.n[2];
+  tm.nzc = (k.n[2] + tm.zc - 1) / tm.zc;
+  const int tblocks = tm.ntx * tm.nty * tm.nzc;
+  const bool tiled = c->variant != 1;
+  const bool dg = c->diagnostics != 0;
+  const T dt = k.dt;
+  const T t0 = (T)time;
+  int rc = COFFEE_OK;
+  for (int st = 0; st < 5; st++) {
+    T* const* in = (st % 2 == 0) ? A : B;
+    T* const* out = (st % 2 == 0) ? B : A;
+    for (int q = 0; q < 7; q++) {
+      io.in[q] = in[q];
+      io.out[q] = out[q];
+    }
+    const T as = (T)As[st], bs = (T)Bs[st];
+    const T ts = t0 + (T)cs[st] * dt;
+    const char* name = st == 0 ? "ez_stage_first" : (st == 4 ? "ez_stage_last" : "ez_stage");
+    rc = timed(c, name, s, [&] {
+      if (st == 0)
+        launch_stage_fl<T, true, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+      else if (st == 4)
+        launch_stage_fl<T, false, true, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+      else
+        launch_stage_fl<T, false, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    });
+    if (rc) return rc;
+  }
+  // Kreiss-Oliger: B -> A
+  for (int q = 0; q < 7; q++) {
+    io.in[q] = B[q];
+    io.out[q] = A[q];
+  }
+  const T t1 = t0 + dt;
+  return timed(c, "ez_ko", s, [&] {
+    if (tiled) {
+      if (star) ez_ko_tiled<T, TY, true><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
+      else ez_ko_tiled<T, TY, false><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
+    } else {
+      if (star) ez_ko_kernel<T, true><<<grid, block, 0, s>>>(k, io, t1);
+      else ez_ko_kernel<T, false><<<grid, block, 0, s>>>(k, io, t1);
+    }
+  });
+}
+
 template <typename T>
 int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s) {
   static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
