@@ -381,11 +381,12 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
                       const coffee_params* params, coffee_ctx** out) {
   if (!out || !grid || !params) return fail(nullptr, COFFEE_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (solver != COFFEE_SOLVER_EZ && solver != COFFEE_SOLVER_EZ_SPHERICAL)
-    return fail(nullptr, COFFEE_ERR_INVALID, "solver variant not available in this build");
+  if (solver != COFFEE_SOLVER_EZ && solver != COFFEE_SOLVER_EZ_SPHERICAL &&
+      solver != COFFEE_SOLVER_GR_EZ)
+    return fail(nullptr, COFFEE_ERR_INVALID, "unknown solver variant");
   if (scalar_bytes != COFFEE_F32 && scalar_bytes != COFFEE_F64)
     return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
-  int rc = solver == COFFEE_SOLVER_EZ ? validate(nullptr, grid, params) : COFFEE_OK;
+  int rc = solver != COFFEE_SOLVER_EZ_SPHERICAL ? validate(nullptr, grid, params) : COFFEE_OK;
   if (rc) return rc;
   coffee_ctx* c = new coffee_ctx();
   c->solver = solver;
@@ -423,6 +424,14 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
       return fail(nullptr, COFFEE_ERR_HIP, m);
     }
   }
+  if (solver == COFFEE_SOLVER_GR_EZ) {
+    int rg = coffee::gr_create(c);
+    if (rg) {
+      std::string m = c->err;
+      coffee_ctx_destroy(c);
+      return fail(nullptr, rg, m);
+    }
+  }
   *out = c;
   return COFFEE_OK;
 }
@@ -458,7 +467,7 @@ int coffee_ez_step(coffee_ctx* c, const coffee_fields* f, double time, void* str
   if (!c || !f) return fail(c, COFFEE_ERR_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
   if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) return coffee::sph_step(c, f, time, s);
-  if (c->solver != COFFEE_SOLVER_EZ) return fail(c, COFFEE_ERR_STATE, "unsupported solver");
+  if (c->solver == COFFEE_SOLVER_GR_EZ) return coffee::gr_step(c, f, time, s);
   return c->scalar == 4 ? ez_step_impl<float>(c, f, time, s) : ez_step_impl<double>(c, f, time, s);
 }
 
@@ -483,11 +492,14 @@ int coffee_total_energy(coffee_ctx* c, void* const comp[3], double* out, void* s
   if (!c || !comp || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
   if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) return coffee::sph_energy(c, comp, out, s);
+  if (c->solver == COFFEE_SOLVER_GR_EZ) return coffee::gr_energy(c, comp, out, s);
   return c->scalar == 4 ? energy_impl<float>(c, comp, out, s) : energy_impl<double>(c, comp, out, s);
 }
 
 int coffee_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], void* stream) {
   if (!c || !f || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  if (c->solver != COFFEE_SOLVER_EZ)
+    return fail(c, COFFEE_ERR_STATE, "residuals are available for the flat solver");
   hipStream_t s = (hipStream_t)stream;
   return c->scalar == 4 ? residuals_impl<float>(c, f, out, s) : residuals_impl<double>(c, f, out, s);
 }

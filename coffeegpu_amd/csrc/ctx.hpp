@@ -109,7 +109,17 @@ int timed(coffee_ctx* c, const char* name, hipStream_t s, F&& launch) {
   return COFFEE_OK;
 }
 
+template <typename T>
+inline FlatK<T>& flat_k(coffee_ctx* c);
+template <>
+inline FlatK<float>& flat_k<float>(coffee_ctx* c) { return c->kf; }
+template <>
+inline FlatK<double>& flat_k<double>(coffee_ctx* c) { return c->kd; }
+
 // Entry points of the other translation units, dispatched from the ABI.
+int gr_create(coffee_ctx* c);
+int gr_step(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s);
+int gr_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s);
 int sph_create(coffee_ctx* c);
 int sph_step(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s);
 int sph_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s);

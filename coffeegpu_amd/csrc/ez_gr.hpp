@@ -1,0 +1,417 @@
+// Kerr-Schild GR EZ kernels for gfx950 (Cartesian Kerr-Schild, M = 1).
+//
+// Semantics: src/algorithms/field_solver_gr_EZ.cu, boundary.cu:7-79 and
+// metric_cks_impl.hpp (the reference's only implementation of this path).
+// Evolved: D (the sim_data "E" arrays), B (upper components) and P.
+//
+// The 3-metric of Kerr-Schild is gamma_ij = delta_ij + 2H l_i l_j with
+// H = r^3 / (r^4 + a^2 z^2), l = ((r x + a y)/(r^2 + a^2), (r y - a x)/(r^2 + a^2), z/r)
+// and |l| = 1, so everything the solver needs follows from (r, s = 2H, l):
+//   sqrt(gamma) = sqrt(1 + s),  alpha = 1/sqrt(1 + s),
+//   gamma_ij v^j = v_i + s l_i (l.v),   gamma^ij w_j = w^i - s/(1+s) l^i (l.w),
+//   beta^i = s/(1+s) l^i.
+// The reference evaluates some 20 metric functions at each use, each
+// recomputing r (metric_cks_impl.hpp:17-181); here one FP64 evaluation of
+// (r, s, l) per grid point feeds all of them, in both Scalar precisions.
+//
+// One step is, per stage: gr_point (E_i, H_i and sqrt(gamma) D, sqrt(gamma) B
+// at every point of the guarded grid, kernel_compute_E_gr/_H_gr .cu:90-186),
+// then gr_stage (rk_step1 + rk_step2 + clean_epar + check_eGTb, and the
+// absorbing layers in the last stage, .cu:188-535, 610-701, boundary.cu:17-79),
+// with U ping-ponging between two buffers; then gr_ko (KO on D, B, P +
+// projections, .cu:537-608).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ez_flat.hpp"
+
+namespace coffee {
+
+template <typename T>
+struct GrK {
+  int dims[3], guard[3], n[3];
+  int sy, sz;
+  double lower[3], delta[3];  // Grid::pos in FP64
+  double a;                   // spin
+  T c12[3];                   // inv_delta / 12
+  T dt, tau, KOfac;
+  int calc_current, divB_clean, clean_ep, check_egb;
+  int pml_any;
+  double xl[3], xh[3];  // outer absorbing-layer bounds (boundary.cu:41-52)
+  double pmlscale;      // pmllen * delta[0] on every axis (boundary.cu:55-60)
+  double sigpml;
+  double r1, r2, dd;    // 0.7 rH, 0.01 rH, 0.2 rH (.cu:625-628)
+  double dt_d;
+};
+
+template <typename T>
+struct GrIO {
+  const T* __restrict__ in[7];
+  T* __restrict__ out[7];
+  T* __restrict__ dU[7];
+  T* __restrict__ Un[7];
+  T* __restrict__ J[3];
+  T* __restrict__ divE;
+  T* __restrict__ divB;
+  T* __restrict__ W[12];  // E_i (3), H_i (3), sqrt(g) D^i (3), sqrt(g) B^i (3)
+};
+
+// s = 2H feeds gamma_ij, alpha and sqrt(gamma); bu = 2r^3 / (2r^3 + r^4 + a^2 z^2)
+// feeds gamma^ij and beta^i.  They are s/(1+s) of each other except where the
+// reference's TINY clamp of r^4 + a^2 z^2 acts (grid points on the ring
+// singularity), where the reference keeps them apart and so does this code.
+struct Cks {
+  double r, s, bu, lx, ly, lz;
+};
+
+// r of metric_cks_impl.hpp:17-24 and the Kerr-Schild H, l at one point.
+__device__ __forceinline__ Cks cks_point(double a, double x, double y, double z) {
+  const double tiny = 1e-11;
+  const double R2 = x * x + y * y + z * z;
+  const double w = R2 - a * a;
+  const double az = a * z;
+  Cks m;
+  m.r = sqrt((w + sqrt(fmax(w * w + 4.0 * (az * az), tiny))) / 2.0);
+  const double r = m.r, r2 = r * r;
+  const double Q = fmax(r2 * r2 + az * az, tiny);
+  const double A = fmax(a * a + r2, tiny);
+  const double r3 = 2.0 * r * r2;
+  m.s = r3 / Q;
+  m.bu = r3 / fmax(r3 + r2 * r2 + az * az, tiny);
+  const double iA = 1.0 / A;
+  m.lx = (r * x + a * y) * iA;
+  m.ly = (r * y - a * x) * iA;
+  m.lz = r > 0.0 ? z / r : 0.0;
+  return m;
+}
+
+template <typename T>
+struct CksT {
+  T s, l[3], alpha, sg, bfac;  // bfac = bu (= s / (1 + s) away from the clamp)
+  double r;
+};
+
+template <typename T>
+__device__ __forceinline__ CksT<T> cks_scalar(const Cks& m) {
+  const double g = 1.0 + m.s;
+  const double sg = sqrt(g);
+  CksT<T> t;
+  t.r = m.r;
+  t.s = (T)m.s;
+  t.l[0] = (T)m.lx;
+  t.l[1] = (T)m.ly;
+  t.l[2] = (T)m.lz;
+  t.sg = (T)sg;
+  t.alpha = (T)(1.0 / sg);
+  t.bfac = (T)m.bu;
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void gr_lower(const CksT<T>& m, const T v[3], T out[3]) {
+  const T lv = m.l[0] * v[0] + m.l[1] * v[1] + m.l[2] * v[2];
+  const T sl = m.s * lv;
+#pragma unroll
+  for (int i = 0; i < 3; i++) out[i] = v[i] + sl * m.l[i];
+}
+
+template <typename T>
+__device__ __forceinline__ T dot3(const T u[3], const T v[3]) {
+  return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+
+template <typename T>
+__device__ __forceinline__ double gr_pos(const GrK<T>& k, int ax, int i) {
+  return k.lower[ax] + k.delta[ax] * (double)(i - k.guard[ax]);
+}
+
+// E_i = alpha gamma_ij D^j + sqrt(g) (beta x B)_i, H_i = alpha gamma_ij B^j
+// - sqrt(g) (beta x D)_i (.cu:111-184), with beta = bu l:
+// E = alpha (D + s l (l.D)) + sqrt(g) bu (l x B), H likewise with B, -D.
+template <typename T>
+__device__ __forceinline__ void gr_point_values(const CksT<T>& m, const T D[3], const T B[3],
+                                                T E[3], T H[3]) {
+  const T* l = m.l;
+  const T slD = m.s * dot3(l, D), slB = m.s * dot3(l, B);
+  const T w = m.sg * m.bfac;
+  const T cB[3] = {l[1] * B[2] - l[2] * B[1], l[2] * B[0] - l[0] * B[2], l[0] * B[1] - l[1] * B[0]};
+  const T cD[3] = {l[1] * D[2] - l[2] * D[1], l[2] * D[0] - l[0] * D[2], l[0] * D[1] - l[1] * D[0]};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    E[i] = m.alpha * (D[i] + slD * l[i]) + w * cB[i];
+    H[i] = m.alpha * (B[i] + slB * l[i]) - w * cD[i];
+  }
+}
+
+// clean_epar then check_eGTb with the metric (.cu:440-535) on D (u[0..2]).
+template <typename T>
+__device__ __forceinline__ void gr_project(const GrK<T>& k, const CksT<T>& m, T u[7]) {
+  const T tiny = TinyOf<T>::v;
+  T D[3] = {u[0], u[1], u[2]};
+  const T B[3] = {u[3], u[4], u[5]};
+  T Bd[3];
+  gr_lower(m, B, Bd);
+  const T B2raw = dot3(B, Bd);
+  if (k.clean_ep) {
+    const T B2 = B2raw < tiny ? tiny : B2raw;
+    const T DB = dot3(D, Bd);
+#pragma unroll
+    for (int i = 0; i < 3; i++) D[i] = D[i] - DB * B[i] / B2;
+  }
+  if (k.check_egb) {
+    T Dd[3];
+    gr_lower(m, D, Dd);
+    T D2 = dot3(D, Dd);
+    if (D2 < tiny) D2 = tiny;
+    const T temp = D2 > B2raw ? sqrt(B2raw / D2) : T(1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) D[i] = temp * D[i];
+  }
+  u[0] = D[0];
+  u[1] = D[1];
+  u[2] = D[2];
+}
+
+template <typename T>
+__device__ __forceinline__ bool gr_in_outer(const GrK<T>& k, double x, double y, double z) {
+  return k.pml_any && (x > k.xh[0] || x < k.xl[0] || y > k.xh[1] || y < k.xl[1] ||
+                       z > k.xh[2] || z < k.xl[2]);
+}
+
+__device__ __forceinline__ double gr_pmlsigma(double x, double xl, double xh, double scale,
+                                              double sig0) {
+  // pmlsigma (boundary.cu:7-15)
+  if (x > xh) {
+    const double v = (x - xh) / scale;
+    return sig0 * v * v * v;
+  }
+  if (x < xl) {
+    const double v = (xl - x) / scale;
+    return sig0 * v * v * v;
+  }
+  return 0.0;
+}
+
+// kernel_boundary_absorbing_thread (boundary.cu:17-79) then
+// kernel_absorbing_inner (.cu:617-701) on D, B; P untouched.
+template <typename T>
+__device__ __forceinline__ void gr_absorb(const GrK<T>& k, const GrIO<T>& io, int c, double x,
+                                          double y, double z, double r, T u[7]) {
+  const double tiny = (double)TinyOf<T>::v;
+  if (gr_in_outer(k, x, y, z)) {
+    const double sig = gr_pmlsigma(x, k.xl[0], k.xh[0], k.pmlscale, k.sigpml) +
+                       gr_pmlsigma(y, k.xl[1], k.xh[1], k.pmlscale, k.sigpml) +
+                       gr_pmlsigma(z, k.xl[2], k.xh[2], k.pmlscale, k.sigpml);
+    if (sig > tiny) {
+      const double e = exp(-sig);
+      const T ex = (T)e, w = (T)((1.0 - e) / sig);
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        const T un = io.Un[f][c];
+        u[f] = ex * un + w * (u[f] - un);
+      }
+    }
+  }
+  if (r < k.r1) {
+    const double v = (k.r1 - r) / k.dd;
+    const double sig = k.sigpml * v * v * v * k.dt_d;
+    if (sig > tiny) {
+      const double e = exp(-sig);
+      const T ex = (T)e, w = (T)((1.0 - e) / sig);
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        const T un = io.Un[f][c];
+        u[f] = ex * un + w * (u[f] - un);
+      }
+    }
+  }
+  if (r < k.r2) {
+#pragma unroll
+    for (int f = 0; f < 6; f++) u[f] = T(0);
+  }
+}
+
+// Every point of the guarded grid: W = (E_i, H_i, sqrt(g) D^i, sqrt(g) B^i).
+template <typename T>
+__global__ void __launch_bounds__(256) gr_point_kernel(const GrK<T> k, const GrIO<T> io) {
+  const int total = k.dims[0] * k.dims[1] * k.dims[2];
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
+    const int i = c % k.dims[0];
+    const int r = c / k.dims[0];
+    const int j = r % k.dims[1];
+    const int kk = r / k.dims[1];
+    const CksT<T> m = cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, i), gr_pos(k, 1, j), gr_pos(k, 2, kk)));
+    const T D[3] = {io.in[0][c], io.in[1][c], io.in[2][c]};
+    const T B[3] = {io.in[3][c], io.in[4][c], io.in[5][c]};
+    T E[3], H[3];
+    gr_point_values(m, D, B, E, H);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      io.W[q][c] = E[q];
+      io.W[3 + q][c] = H[q];
+      io.W[6 + q][c] = m.sg * D[q];
+      io.W[9 + q][c] = m.sg * B[q];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T gr_d1(const T* __restrict__ f, int c, int s, T c12) {
+  return fma(T(8), f[c + s] - f[c - s], f[c - 2 * s] - f[c + 2 * s]) * c12;
+}
+
+// One LSRK stage at interior cell (i, j, kk) (rk_step1 .cu:188-373, rk_step2
+// .cu:375-438, projections .cu:440-535, absorbing in the last stage).
+template <typename T, bool FIRST, bool LAST, bool DIAG>
+__device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io, int i, int j,
+                                              int kk, int c, T As, T Bs) {
+  const int sy = k.sy, sz = k.sz;
+  const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
+  const CksT<T> m = cks_scalar<T>(cks_point(k.a, x, y, z));
+  const T tiny = TinyOf<T>::v;
+  const T* const* W = (const T* const*)io.W;
+  // curl numerators of E_i, H_i; rot = curl / sqrt(g) = alpha * curl
+  const T rotE[3] = {m.alpha * (gr_d1(W[2], c, sy, k.c12[1]) - gr_d1(W[1], c, sz, k.c12[2])),
+                     m.alpha * (gr_d1(W[0], c, sz, k.c12[2]) - gr_d1(W[2], c, 1, k.c12[0])),
+                     m.alpha * (gr_d1(W[1], c, 1, k.c12[0]) - gr_d1(W[0], c, sy, k.c12[1]))};
+  const T rotH[3] = {m.alpha * (gr_d1(W[5], c, sy, k.c12[1]) - gr_d1(W[4], c, sz, k.c12[2])),
+                     m.alpha * (gr_d1(W[3], c, sz, k.c12[2]) - gr_d1(W[5], c, 1, k.c12[0])),
+                     m.alpha * (gr_d1(W[4], c, 1, k.c12[0]) - gr_d1(W[3], c, sy, k.c12[1]))};
+  // div4 (.cu:44-88): d_i(sqrt(g) f^i) / sqrt(g)
+  const T divD = m.alpha * (gr_d1(W[6], c, 1, k.c12[0]) + gr_d1(W[7], c, sy, k.c12[1]) +
+                            gr_d1(W[8], c, sz, k.c12[2]));
+  const T divB = m.alpha * (gr_d1(W[9], c, 1, k.c12[0]) + gr_d1(W[10], c, sy, k.c12[1]) +
+                            gr_d1(W[11], c, sz, k.c12[2]));
+  T u[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) u[f] = io.in[f][c];
+  const T D[3] = {u[0], u[1], u[2]};
+  const T B[3] = {u[3], u[4], u[5]};
+  T Bd[3], Dd[3];
+  gr_lower(m, B, Bd);
+  gr_lower(m, D, Dd);
+  T J[3] = {T(0), T(0), T(0)};
+  if (k.calc_current) {
+    T B2 = dot3(B, Bd);
+    if (B2 < tiny) B2 = tiny;
+    const T Jp = dot3(Bd, rotH) - dot3(Dd, rotE);
+    const T E[3] = {W[0][c], W[1][c], W[2][c]};
+    const T rho = divD * m.alpha;
+    J[0] = (rho * (E[1] * Bd[2] - E[2] * Bd[1]) + Jp * B[0]) / B2;
+    J[1] = (rho * (E[2] * Bd[0] - E[0] * Bd[2]) + Jp * B[1]) / B2;
+    J[2] = (rho * (E[0] * Bd[1] - E[1] * Bd[0]) + Jp * B[2]) / B2;
+  }
+  const T* P = io.in[6];
+  const T gP[3] = {gr_d1(P, c, 1, k.c12[0]), gr_d1(P, c, sy, k.c12[1]), gr_d1(P, c, sz, k.c12[2])};
+  const T lgP = dot3(m.l, gP);
+  const T beta[3] = {m.bfac * m.l[0], m.bfac * m.l[1], m.bfac * m.l[2]};
+  const T shape = T(0.5) * (T(1) - tanh((T)(m.r - 5.0)));
+  T du[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) du[f] = FIRST ? T(0) : io.dU[f][c];
+  const T dt = k.dt;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    if (k.divB_clean) {
+      const T Pu = gP[q] - m.bfac * m.l[q] * lgP;
+      du[3 + q] = As * du[3 + q] + dt * (-rotE[q] + (-m.alpha * Pu + beta[q] * divB) * shape);
+    } else {
+      du[3 + q] = As * du[3 + q] - dt * rotE[q];
+    }
+    du[q] = As * du[q] + dt * (rotH[q] - J[q]);
+  }
+  du[6] = As * du[6] + dt * (-m.alpha * divB + dot3(beta, gP)) - m.alpha * u[6] / k.tau;
+  if (FIRST && (gr_in_outer(k, x, y, z) || m.r < k.r1)) {
+#pragma unroll
+    for (int f = 0; f < 7; f++) io.Un[f][c] = u[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 7; f++) u[f] = u[f] + Bs * du[f];
+  gr_project(k, m, u);
+  if (LAST) gr_absorb(k, io, c, x, y, z, m.r, u);
+#pragma unroll
+  for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
+  if (!LAST) {
+#pragma unroll
+    for (int f = 0; f < 7; f++) io.dU[f][c] = du[f];
+  }
+  if (DIAG) {
+    io.J[0][c] = J[0];
+    io.J[1][c] = J[1];
+    io.J[2][c] = J[2];
+    io.divE[c] = divD;
+    io.divB[c] = divB;
+  }
+}
+
+template <typename T, bool FIRST, bool LAST, bool DIAG>
+__global__ void __launch_bounds__(256) gr_stage_kernel(const GrK<T> k, const GrIO<T> io, const T As,
+                                                       const T Bs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int kk = blockIdx.z;
+  if (i >= k.n[0] || j >= k.n[1]) return;
+  const int ai = i + k.guard[0], aj = j + k.guard[1], ak = kk + k.guard[2];
+  const int c = ai + aj * k.sy + ak * k.sz;
+  gr_stage_cell<T, FIRST, LAST, DIAG>(k, io, ai, aj, ak, c, As, Bs);
+}
+
+// KO on D, B, P (.cu:537-608) then the projections (.cu:902-904).
+template <typename T>
+__global__ void __launch_bounds__(256) gr_ko_kernel(const GrK<T> k, const GrIO<T> io) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int kk = blockIdx.z;
+  if (i >= k.n[0] || j >= k.n[1]) return;
+  const int ai = i + k.guard[0], aj = j + k.guard[1], ak = kk + k.guard[2];
+  const int c = ai + aj * k.sy + ak * k.sz;
+  const int ox[6] = {-3, -2, -1, 1, 2, 3};
+  const int oy[6] = {-3 * k.sy, -2 * k.sy, -k.sy, k.sy, 2 * k.sy, 3 * k.sy};
+  const int oz[6] = {-3 * k.sz, -2 * k.sz, -k.sz, k.sz, 2 * k.sz, 3 * k.sz};
+  T u[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) {
+    const T* p = io.in[f];
+    const T t = d6(p, c, ox) + d6(p, c, oy) + d6(p, c, oz);
+    u[f] = fma(t, k.KOfac, p[c]);
+  }
+  const CksT<T> m = cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak)));
+  gr_project(k, m, u);
+#pragma unroll
+  for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
+}
+
+// Field energy sum over the interior outside the absorbing layers:
+// sum gamma_ij f^i f^j sqrt(gamma) (no reference counterpart: the reference
+// declares field_solver_gr_EZ::total_energy but never defines it).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gr_energy_kernel(const GrK<T> k, const T* __restrict__ f0, const T* __restrict__ f1,
+                     const T* __restrict__ f2, double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const int total = k.n[0] * k.n[1] * k.n[2];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = t % k.n[0] + k.guard[0];
+    const int r = t / k.n[0];
+    const int j = r % k.n[1] + k.guard[1];
+    const int kk = r / k.n[1] + k.guard[2];
+    const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
+    const Cks m = cks_point(k.a, x, y, z);
+    if (gr_in_outer(k, x, y, z) || m.r < k.r1) continue;
+    const int c = i + j * k.sy + kk * k.sz;
+    const double v[3] = {(double)f0[c], (double)f1[c], (double)f2[c]};
+    const double lv = m.lx * v[0] + m.ly * v[1] + m.lz * v[2];
+    acc += ((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + m.s * lv * lv) * sqrt(1.0 + m.s);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace coffee

@@ -195,3 +195,37 @@ def periodic_fill(U: np.ndarray, grid: Grid, periodic) -> np.ndarray:
 def stack(fields: Dict[str, np.ndarray]) -> np.ndarray:
     """(7, k, j, i) array in the field order Ex..Bz, P."""
     return np.stack([fields[n] for n in FIELDS])
+
+
+def cks_metric_np(a: float, x, y, z):
+    """Cartesian Kerr-Schild (M = 1) in FP64: r, s = 2H and l on numpy arrays
+    (the quantities of src/algorithms/metric_cks_impl.hpp; gamma_ij =
+    delta_ij + s l_i l_j, sqrt(gamma) = sqrt(1 + s))."""
+    tiny = 1e-11
+    x, y, z = (np.asarray(v, dtype=np.float64) for v in (x, y, z))
+    w = x * x + y * y + z * z - a * a
+    r = np.sqrt((w + np.sqrt(np.maximum(w * w + 4.0 * (a * z) ** 2, tiny))) / 2.0)
+    Q = np.maximum(r ** 4 + (a * z) ** 2, tiny)
+    A = np.maximum(a * a + r * r, tiny)
+    s = 2.0 * r ** 3 / Q
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lz = np.where(r > 0, z / np.where(r > 0, r, 1.0), 0.0)
+    return r, s, (r * x + a * y) / A, (r * y - a * x) / A, lz
+
+
+def wald_uniform(grid: Grid, a: float, noise: float = 0.0, seed: int = 0, dtype=np.float64):
+    """Initial data of src/user_wald2.hpp:13-49: B = (0, 0, 1/sqrt(gamma)),
+    D = 0, P = 0 on every node including guards, plus `noise` times seeded
+    standard-normal white noise on the three B components."""
+    X, Y, Z = _coords64(grid)
+    _, s, _, _, _ = cks_metric_np(a, X, Y, Z)
+    shape = grid.shape
+    zero = np.zeros(shape)
+    Bz = 1.0 / np.sqrt(1.0 + s)
+    comp = {"Ex": zero, "Ey": zero, "Ez": zero, "Bx": zero.copy(), "By": zero.copy(), "Bz": Bz,
+            "P": zero}
+    if noise:
+        rng = np.random.default_rng(seed)
+        for key in ("Bx", "By", "Bz"):
+            comp[key] = comp[key] + noise * rng.standard_normal(shape)
+    return _pack(grid, dtype, **comp)

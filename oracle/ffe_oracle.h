@@ -127,6 +127,21 @@ int ffe_sph_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7]
 int ffe_sph_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
                          double *const L[6], double As, double Bs);
 
+/* ---- Kerr-Schild GR EZ (field_solver_gr_EZ.cu), FP64 ------------------- */
+
+/* U = D (the evolved "E" arrays of sim_data), B, P; dU = dD, dB, dP;
+ * Un = Dtmp, Btmp, Ptmp (step-start copies, then KO scratch);
+ * EH = Ed, Hd scratch (6 arrays).  `time` is unused, as in the reference. */
+int ffe_gr_evolve_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                      double *const Un[7], double *const EH[6], const ffe_diag_f64 *diag,
+                      double time, int nsteps);
+int ffe_gr_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
+                        double *const EH[6], const ffe_diag_f64 *diag, double As, double Bs);
+int ffe_gr_eh_f64(const ffe_params *p, double *const U[7], double *const EH[6]);
+/* gd11 gd12 gd13 gd22 gd23 gd33, gu11 .. gu33, bd1..3, bu1..3, alpha,
+ * sqrt(gamma), r at one point (metric_cks_impl.hpp). */
+void ffe_gr_metric_f64(double a, double x, double y, double z, double out[21]);
+
 /* Closed-form helpers (src/algorithms/pulsar_impl.hpp) in double. */
 double ffe_dipole2(double x, double y, double z, double p1, double p2, double p3,
                    double phase, int n);

@@ -245,3 +245,69 @@ def sph_energy(sp, F: np.ndarray) -> float:
     fn = L.ffe_sph_energy_f64 if F.dtype == np.float64 else L.ffe_sph_energy_f32
     fn.restype = ctypes.c_double
     return fn(ctypes.byref(make_params(sp)), _ptrs(list(np.ascontiguousarray(F)), ct))
+
+
+class GrState:
+    """Host state of the Kerr-Schild solver (FP64): U = D, B, P; dU; Un; EH."""
+
+    def __init__(self, U: np.ndarray):
+        self.U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        self.dU = np.zeros_like(self.U)
+        self.Un = np.zeros_like(self.U)
+        self.EH = np.zeros((6,) + self.U.shape[1:], dtype=np.float64)
+
+
+def _gr_diag(diag):
+    if diag is None:
+        return None
+    return ctypes.byref(_make_diag(diag, ctypes.c_double, np.float64))
+
+
+def gr_evolve(sp, state: GrState, nsteps: int, diag=None):
+    L = lib()
+    p = make_params(sp)
+    ct = ctypes.c_double
+    fn = L.ffe_gr_evolve_f64
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.byref(p), _ptrs(list(state.U), ct), _ptrs(list(state.dU), ct),
+            _ptrs(list(state.Un), ct), _ptrs(list(state.EH), ct), _gr_diag(diag),
+            ctypes.c_double(0.0), ctypes.c_int(nsteps))
+    if rc != 0:
+        raise RuntimeError(f"oracle gr_evolve failed rc={rc}")
+    return state
+
+
+def gr_rk_stage(sp, state: GrState, As: float, Bs: float, diag=None):
+    L = lib()
+    ct = ctypes.c_double
+    fn = L.ffe_gr_rk_stage_f64
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.byref(make_params(sp)), _ptrs(list(state.U), ct), _ptrs(list(state.dU), ct),
+            _ptrs(list(state.EH), ct), _gr_diag(diag), ctypes.c_double(As), ctypes.c_double(Bs))
+    if rc != 0:
+        raise RuntimeError(f"oracle gr_rk_stage failed rc={rc}")
+    return state
+
+
+def gr_eh(sp, U: np.ndarray) -> np.ndarray:
+    L = lib()
+    ct = ctypes.c_double
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    EH = np.zeros((6,) + U.shape[1:], dtype=np.float64)
+    L.ffe_gr_eh_f64.restype = ctypes.c_int
+    L.ffe_gr_eh_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ct), _ptrs(list(EH), ct))
+    return EH
+
+
+GR_METRIC_NAMES = ("gd11", "gd12", "gd13", "gd22", "gd23", "gd33", "gu11", "gu12", "gu13",
+                   "gu22", "gu23", "gu33", "bd1", "bd2", "bd3", "bu1", "bu2", "bu3", "alpha",
+                   "sg", "r")
+
+
+def gr_metric(a: float, x: float, y: float, z: float) -> dict:
+    L = lib()
+    out = (ctypes.c_double * 21)()
+    L.ffe_gr_metric_f64.restype = None
+    L.ffe_gr_metric_f64(ctypes.c_double(a), ctypes.c_double(x), ctypes.c_double(y),
+                        ctypes.c_double(z), out)
+    return dict(zip(GR_METRIC_NAMES, list(out)))
