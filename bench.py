@@ -53,6 +53,11 @@ def parse():
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0 = auto)")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=10)
+    ap.add_argument("--workload", choices=["alfven3d", "sph-sweep", "gr-wald"], default="alfven3d",
+                    help="alfven3d: the headline flat 512^3 run (default); sph-sweep: batched 2D "
+                         "pulsar runs; gr-wald: Kerr-Schild Wald problem")
+    ap.add_argument("--runs", type=int, default=256, help="sph-sweep: number of runs")
+    ap.add_argument("--spin", type=float, default=0.9, help="gr-wald: black-hole spin a")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a plain device copy of this many MiB (3 launches), for "
                          "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes")
@@ -130,6 +135,179 @@ def cpu_baseline(args, dtype):
             "kind": "port", "sample": sample + f"; project CPU oracle, OpenMP {threads} threads"}
 
 
+def _kernel_table(timing, scalars, esz, cells):
+    return {k: {"launches": c, "avg_ms": m / c,
+                "algorithmic_GBps": scalars[k] * esz * cells / (m / c * 1e-3) / 1e9
+                if k in scalars else None}
+            for k, (c, m) in timing.items()}
+
+
+def sph_sweep_params(rng, nr: int, nth: int):
+    """One run of the batched 2D pulsar sweep: star R = 1, b0 = 1, x = ln r from
+    -0.1 to ln 60, theta in [0, pi], absorbing layer r > 48, R_LC ~ U[3, 20]."""
+    import math
+
+    from coffeegpu_amd.params import SimParams
+
+    r_out, r_abs = 60.0, 48.0
+    x0 = -0.1
+    size_x = math.log(r_out) - x0
+    dx, dth = size_x / nr, math.pi / nth
+    rlc = float(rng.uniform(3.0, 20.0))
+    pml = int(math.ceil((math.log(r_out) - math.log(r_abs)) / dx))
+    return SimParams(dt=0.4 * min(dx, dth), N=[nr, nth, 1], guard=[3, 4, 0], lower=[x0, 0.0, 0.0],
+                     size=[size_x, math.pi, 0.0], nodes=[1, 1, 1],
+                     periodic_boundary=[False, False, False], pml=[pml, 0, 0], pmllen=10,
+                     sigpml=1.0, divB_clean=True, ch2=1.0, tau=2.0, KOeps=0.1, KO_geometry=True,
+                     use_edotb_damping=False, calc_current=True, clean_ep=True, check_egb=True,
+                     problem=0, radius=1.0, omega=1.0 / rlc, b0=1.0, dw0=0.0, dw1=0.0), rlc
+
+
+# sph stage kernels move the same scalars per cell as the flat ones
+SPH_SCALARS = {"sph_stage": 28, "sph_stage_first": 21, "sph_stage_last": 21, "sph_ko": 14}
+
+
+def run_sph_sweep(args):
+    import torch
+
+    from coffeegpu_amd import initial_data as ic
+    from coffeegpu_amd.solver import SimData, SimEnvironment
+    from coffeegpu_amd.spherical import SphBatch
+
+    rng = np.random.default_rng(args.seed)
+    shapes = [(512, 256), (1024, 512), (2048, 1024)]
+    envs, datas, cells, rlcs = [], [], 0, []
+    t0 = time.perf_counter()
+    for r in range(args.runs):
+        nr, nth = shapes[int(rng.integers(0, len(shapes)))]
+        sp, rlc = sph_sweep_params(rng, nr, nth)
+        env = SimEnvironment(sp, np.float32, "cuda:0")
+        d = SimData(env)
+        f = ic.pulsar_dipole_sph(env.grid(), sp, dtype=np.float32)
+        for name, arr in zip(ic.FIELDS, d.state_arrays()):
+            arr.set(f[name])
+        envs.append(env)
+        datas.append(d)
+        cells += nr * nth
+        rlcs.append(rlc)
+    b = SphBatch(envs, datas, nsteps=[0] * args.runs)
+    best, tile_ms = b.autotune(trials=3)
+    b.use_graph(True)
+    print(f"[bench] sph-sweep {args.runs} runs, {cells / 1e6:.1f} M cells, set-up "
+          f"{time.perf_counter() - t0:.1f}s, tile {best} {tile_ms}", file=sys.stderr, flush=True)
+    b.step(args.warmup)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    b.step(args.steps)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_total = start.elapsed_time(stop)
+    ms_per_step = ms_total / args.steps
+    value = cells / (ms_per_step * 1e-3)
+    # per-kernel durations: direct launches with events (graphs are off while timing)
+    b.set_timing(True)
+    b.step(3)
+    torch.cuda.synchronize()
+    timing = b.get_timing()
+    b.set_timing(False)
+    cnt, ms = timing["sph_stage"]
+    bytes_per_launch = SPH_SCALARS["sph_stage"] * 4 * cells
+    achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+    lum = b.luminosity(1.5 * np.asarray(rlcs))
+    out = {
+        "metric": "cell-updates/s (batched 2D log-spherical pulsar sweep, 1 GPU)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (rotating dipoles, seeded grids and R_LC)",
+        "config": {"workload": f"sph-sweep {args.runs} runs, grids from {shapes}, R_LC~U[3,20], "
+                               "r_out 60, absorbing r>48, hipGraph replay, autotuned tile",
+                   "cells": cells, "tile": best, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_stage",
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "step_algorithmic_GBps": SCALARS_PER_CELL_UPDATE * 4 * value / 1e9,
+        "kernels": _kernel_table(timing, SPH_SCALARS, 4, cells),
+        "luminosity_1p5RLC_median": float(np.median(lum)),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    b.close()
+
+
+GR_SCALARS = {"gr_stage": 28, "gr_stage_first": 21, "gr_stage_last": 21, "gr_ko": 14,
+              "gr_point": 18}
+
+
+def run_gr(args):
+    import torch
+
+    from coffeegpu_amd import initial_data as ic
+    from coffeegpu_amd.gr import FieldSolverGREZ
+    from coffeegpu_amd.params import SimParams
+    from coffeegpu_amd.solver import SimData, SimEnvironment
+
+    n = args.n if args.n != 512 else 384
+    dtype = np.float32 if args.dtype == "fp32" else np.float64
+    delta = 40.0 / n
+    sp = SimParams(dt=0.4 * delta, N=[n, n, n], guard=[3, 3, 3], lower=[-20.0] * 3,
+                   size=[40.0] * 3, nodes=[1, 1, 1], periodic_boundary=[False] * 3,
+                   pml=[32, 32, 32], pmllen=1, sigpml=10.0, divB_clean=True, ch2=1.0, tau=2.0,
+                   KOeps=0.1, use_edotb_damping=False, calc_current=True, clean_ep=True,
+                   check_egb=True, problem=0, a=args.spin)
+    env = SimEnvironment(sp, dtype, "cuda:0")
+    t0 = time.perf_counter()
+    f = ic.wald_uniform(env.grid(), sp.a, noise=1e-3, seed=args.seed, dtype=dtype)
+    data = SimData(env)
+    for name, arr in zip(ic.FIELDS, data.state_arrays()):
+        arr.set(f[name])
+    del f
+    solver = FieldSolverGREZ(data, env)
+    print(f"[bench] gr-wald {n}^3 a={sp.a} set-up {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        solver.evolve_fields(0.0)
+    torch.cuda.synchronize()
+    solver.set_timing(True)
+    solver.reset_timing()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        solver.evolve_fields(0.0)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_per_step = start.elapsed_time(stop) / args.steps
+    timing = solver.get_timing()
+    cells = n ** 3
+    value = cells / (ms_per_step * 1e-3)
+    esz = np.dtype(dtype).itemsize
+    cnt, ms = timing["gr_stage"]
+    bytes_per_launch = GR_SCALARS["gr_stage"] * esz * cells
+    achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+    W = solver.total_energy(data.B)
+    out = {
+        "metric": "cell-updates/s (Kerr-Schild GR Wald problem, 1 GPU)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
+        "config": {"workload": f"gr-wald {n}^3 box [-20,20]^3 a={sp.a}, pml 32 (sigpml 10, "
+                               "pmllen 1), horizon excision, FP64 metric on the fly",
+                   "cells": cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gr_stage",
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "kernels": _kernel_table(timing, GR_SCALARS, esz, cells),
+        "energy_B_after": W,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    solver.close()
+
+
 def main():
     args = parse()
     if args.copy_calibration_mib:
@@ -140,6 +318,10 @@ def main():
                                    "part of it (the halo exchange is the single-GPU periodic "
                                    "fill)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "sph-sweep":
+        return run_sph_sweep(args)
+    if args.workload == "gr-wald":
+        return run_gr(args)
     import torch
 
     from coffeegpu_amd import initial_data as ic
@@ -192,10 +374,7 @@ def main():
     bytes_per_launch = SCALARS_PER_CELL["ez_stage"] * esz * cells
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = load_traffic(f"ez_stage_{n}_{args.dtype}_traffic.json")
-    kernels = {k: {"launches": c, "avg_ms": m / c,
-                   "algorithmic_GBps": SCALARS_PER_CELL.get(k, 0) * esz * cells / (m / c * 1e-3) / 1e9
-                   if k in SCALARS_PER_CELL else None}
-               for k, (c, m) in timing.items()}
+    kernels = _kernel_table(timing, SCALARS_PER_CELL, esz, cells)
     step_gbs = SCALARS_PER_CELL_UPDATE * esz * value / 1e9
 
     cpu = None

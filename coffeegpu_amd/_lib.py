@@ -94,6 +94,8 @@ SIGNATURES = {
     "coffee_sph_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
     "coffee_sph_batch_use_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "coffee_sph_batch_luminosity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p]),
     "coffee_sph_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,

@@ -310,6 +310,24 @@ SphState<T>* sph_state(coffee_ctx* c) {
   return static_cast<SphState<T>*>(c->state);
 }
 
+template <typename T>
+int luminosity_impl(coffee_ctx* c, const double* radius, double* out, hipStream_t s) {
+  SphState<T>* S = sph_state<T>(c);
+  const int nr = (int)S->h_runs.size();
+  double* d = nullptr;
+  HIP_TRY(c, hipMalloc(&d, sizeof(double) * 2 * nr));
+  hipError_t e = hipMemcpyAsync(d, radius, sizeof(double) * nr, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    sph_luminosity_kernel<T><<<nr, 256, 0, s>>>(S->d_runs, d, d + nr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d + nr, sizeof(double) * nr, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, COFFEE_ERR_HIP, std::string("luminosity: ") + hipGetErrorString(e));
+  return COFFEE_OK;
+}
+
 }  // namespace
 
 int sph_create(coffee_ctx* c) {
@@ -552,6 +570,14 @@ int coffee_sph_batch_use_graph(coffee_batch* c, int enable) {
   if (c->scalar == 4) static_cast<SphState<float>*>(c->state)->use_graph = enable != 0;
   else static_cast<SphState<double>*>(c->state)->use_graph = enable != 0;
   return COFFEE_OK;
+}
+
+int coffee_sph_batch_luminosity(coffee_batch* c, const double* radius, double* out, void* stream) {
+  if (!c || !c->state || c->solver != COFFEE_SOLVER_EZ_SPHERICAL || !radius || !out)
+    return fail(c, COFFEE_ERR_INVALID, "not a spherical batch or null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return c->scalar == 4 ? luminosity_impl<float>(c, radius, out, s)
+                        : luminosity_impl<double>(c, radius, out, s);
 }
 
 int coffee_sph_batch_status(coffee_batch* c, int* active, double* t, int* step) {

@@ -442,4 +442,44 @@ __global__ void sph_advance_kernel(SphRun<T>* __restrict__ runs, int nruns) {
   if (R.nsteps > 0 && R.step >= R.nsteps) R.active = 0;
 }
 
+// Spin-down luminosity through the sphere r = rad[run] (a diagnostic of this
+// project; the reference has none): L = 2 pi r^2 sum_j S_r sin(theta_j) dtheta,
+// S_r = E_th B_ph - E_ph B_th the radial Poynting flux in orthonormal
+// components (E_th = r E^theta, E_ph = r sin(theta) E^phi of the upper
+// components stored for x = ln r), r^2 S_r linearly interpolated in x between
+// the two nodes around ln(rad).  One workgroup per run; out[run] in double.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    sph_luminosity_kernel(const SphRun<T>* __restrict__ runs, const double* __restrict__ rad,
+                          double* __restrict__ out) {
+  __shared__ double red[256];
+  const SphRun<T>& R = runs[blockIdx.x];
+  const double fi = (log(rad[blockIdx.x]) - (double)R.x0) / (double)R.dx + R.g0;
+  int i0 = (int)floor(fi);
+  i0 = i0 < R.g0 ? R.g0 : (i0 > R.g0 + R.n0 - 2 ? R.g0 + R.n0 - 2 : i0);
+  double w = fi - i0;
+  w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+  double acc = 0.0;
+  for (int j = R.g1 + threadIdx.x; j < R.g1 + R.n1; j += blockDim.x) {
+    const double s = (double)R.sth[j];
+    double f2[2];
+    for (int q = 0; q < 2; q++) {
+      const int i = i0 + q;
+      const int c = i + j * R.dims0;
+      const double r = (double)R.r[i];
+      const double Eth = r * (double)R.U[0][1][c], Eph = r * s * (double)R.U[0][2][c];
+      const double Bth = r * (double)R.U[0][4][c], Bph = r * s * (double)R.U[0][5][c];
+      f2[q] = r * r * (Eth * Bph - Eph * Bth);
+    }
+    acc += ((1.0 - w) * f2[0] + w * f2[1]) * s;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * M_PI * red[0] * (double)R.dy;
+}
+
 }  // namespace coffee

@@ -168,6 +168,16 @@ class SphBatch:
                    self._ctx)
         return {names[i].value.decode(): (cnt[i], ms[i]) for i in range(n.value)}
 
+    def luminosity(self, radius) -> np.ndarray:
+        """Spin-down luminosity of every run through r = radius (scalar or one
+        per run)."""
+        r = np.broadcast_to(np.asarray(radius, dtype=np.float64), (self.n,)).copy()
+        out = np.zeros(self.n, dtype=np.float64)
+        _lib.check(self._lib.coffee_sph_batch_luminosity(
+            self._ctx, r.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+            self._s()), self._ctx)
+        return out
+
     def status(self):
         act = (ctypes.c_int * self.n)()
         t = (ctypes.c_double * self.n)()

@@ -200,6 +200,11 @@ int coffee_sph_batch_autotune(coffee_batch *b, int trials, void *stream, int *be
 int coffee_sph_batch_use_graph(coffee_batch *b, int enable);
 /* Per-run active flag, time and step count (synchronises the device). */
 int coffee_sph_batch_status(coffee_batch *b, int *active, double *t, int *step);
+/* Spin-down luminosity of every run through the sphere r = radius[run]:
+ * L = 2 pi r^2 \int (E x B)_r sin(theta) dtheta (orthonormal components, the
+ * units of the fields; interpolated linearly in ln r).  No counterpart in the
+ * reference; synchronous on `stream`, out[run] in double. */
+int coffee_sph_batch_luminosity(coffee_batch *b, const double *radius, double *out, void *stream);
 
 /* Fill n scalars at device pointer p with `value` (asynchronous).  Replaces
  * multi_array::assign_dev and its assign_single_value kernel

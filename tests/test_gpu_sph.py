@@ -153,3 +153,34 @@ def test_autotuner_picks_a_tile_and_keeps_results():
     for d, U0, sp in zip(datas, U0s, sps):
         assert np.array_equal(d.get_state(), run_single(sp, U0, dtype, 3))
     b.close()
+
+
+def test_luminosity_closed_form_and_vacuum():
+    """E^theta = sin/r^2, B^phi = 1/r^2 give r^2 S_r = sin^2(theta) and
+    L = 2 pi int sin^3 = 8 pi / 3 at every radius; a static vacuum dipole
+    (E = 0) radiates nothing."""
+    dtype = np.float64
+    envs, datas, U0s, sps = _batch_setup(dtype, [dict(n=(64, 48), omega=0.0, pml=0),
+                                                 dict(n=(64, 48), omega=0.0, pml=0)])
+    g = envs[0].grid()
+    X, Y, _ = (np.array(v) for v in np.broadcast_arrays(*g.coords()))
+    r, th = np.exp(X), Y
+    U = np.zeros_like(U0s[0])
+    U[1] = np.sin(th) / r ** 2
+    U[5] = 1.0 / r ** 2
+    datas[0].set_state(U)
+    b = SphBatch(envs, datas, nsteps=[0, 0])
+    L = b.luminosity([2.0, 3.3])
+    assert abs(L[0] - 8 * np.pi / 3) <= 1e-6 * 8 * np.pi / 3
+    assert L[1] == 0.0
+    b.close()
+
+
+def test_luminosity_of_rotator_is_positive():
+    dtype = np.float32
+    envs, datas, U0s, sps = _batch_setup(dtype, [dict(n=(96, 48), omega=0.3, pml=8)])
+    b = SphBatch(envs, datas, nsteps=[0])
+    b.step(200)
+    L = b.luminosity(2.0)
+    assert np.isfinite(L).all() and L[0] > 0
+    b.close()
