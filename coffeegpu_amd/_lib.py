@@ -58,7 +58,10 @@ class CoffeeFields(ctypes.Structure):
                 ("Bbg", _P3), ("J", _P3), ("divE", ctypes.c_void_p), ("divB", ctypes.c_void_p)]
 
 
-# every symbol include/coffee_hip.h declares, with its ctypes signature
+_V, _I, _L = ctypes.c_void_p, ctypes.c_int, ctypes.c_long
+_F, _D = ctypes.c_float, ctypes.c_double
+
+# every symbol include/*.h declares, with its ctypes signature
 SIGNATURES = {
     "coffee_abi_version": (ctypes.c_int, []),
     "coffee_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
@@ -98,6 +101,26 @@ SIGNATURES = {
                                                     ctypes.c_void_p, ctypes.c_void_p]),
     "coffee_sph_batch_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
+    # include/coffee_unet.h
+    "coffee_unet_conv3x3": (ctypes.c_int, [_V, _V, _V, _V, _V, _I, _I, _I, _I, _I, _I, _I, _V]),
+    "coffee_unet_conv3x3_wgrad": (ctypes.c_int, [_V, _V, _V, _I, _V, _V, _I, _I, _I, _I, _I, _V]),
+    "coffee_unet_combine": (ctypes.c_int, [_V, _I, _I, _V, _I, _I, _V, _I, _I, _V, _I, _I, _I, _L,
+                                           _V]),
+    "coffee_unet_pool2": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
+    "coffee_unet_pool2_bwd": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
+    "coffee_unet_up2": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
+    "coffee_unet_up2_bwd": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
+    "coffee_unet_mse": (ctypes.c_int, [_V, _I, _V, _I, _L, _V, ctypes.POINTER(ctypes.c_double),
+                                       _V]),
+    "coffee_unet_adam": (ctypes.c_int, [_V, _V, _V, _V, _L, _F, _F, _F, _F, _I, _V, _V, _I, _I,
+                                        _V]),
+    "coffee_unet_weights_bf16": (ctypes.c_int, [_V, _V, _V, _I, _I, _V]),
+    "coffee_unet_to_bf16": (ctypes.c_int, [_V, _V, _L, _V]),
+    "coffee_unet_from_bf16": (ctypes.c_int, [_V, _V, _L, _V]),
+    "coffee_unet_pack": (ctypes.c_int, [_V * 6, _I, _I, _I, _I, _I, _I, _D, _D, _D, _D, _D, _V, _I,
+                                        _I, _I, _V]),
+    "coffee_unet_unpack": (ctypes.c_int, [_V, _I, _I, _V * 6, _I, _I, _I, _I, _I, _I, _D, _D, _D,
+                                          _D, _V]),
     "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_void_p]),
     "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcoffee_hip.so")
 
-HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip"]
+HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip", "unet.hip"]
 HOST_SOURCES = []  # C++ host surface, added when present
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.

@@ -1,0 +1,568 @@
+// U-Net building blocks for gfx950 (include/coffee_unet.h): BF16 activations
+// and weights, FP32 accumulation on the matrix cores.
+//
+// The 3x3 convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16: the
+// M dimension is 16 pixels per wavefront (4 wavefronts = 64 pixels per
+// workgroup), N is 16*NB output channels, K runs over the 9 taps times the
+// input channels in steps of 16.  NHWC with channels padded to 16 makes every
+// A and B fragment one 8-byte load per lane (4 consecutive BF16 channels).
+// Operand layout of the 16x16x16 instruction (wave64):
+//   A[i][k] from lane i + 16*(k/4), element k%4;
+//   B[k][j] from lane j + 16*(k/4), element k%4;
+//   D[i][j] in lane j + 16*(i/4), element i%4.
+// The backward data pass is the same kernel with the taps mirrored and the
+// transposed weights.  The weight gradient reduces over pixels (K = pixels),
+// split into ranges whose FP32 partial tiles are summed in a second pass, so
+// results do not depend on scheduling.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "coffee_hip.h"
+#include "coffee_unet.h"
+#include "ctx.hpp"
+
+namespace coffee {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+__device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (bf16)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16)(u >> 16);
+}
+
+int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return COFFEE_OK;
+}
+
+struct ConvArgs {
+  const bf16* in;
+  const bf16* w;
+  const float* bias;
+  bf16* out;
+  float* out32;
+  int N, H, W, Cin, Cout, flip, relu;
+};
+
+template <int NB>
+__global__ void __launch_bounds__(256) conv3x3_mfma(const ConvArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const long P = (long)a.N * a.H * a.W;
+  const long pix0 = ((long)blockIdx.x * 4 + wave) * 16;
+  const int co0 = blockIdx.y * 16 * NB;
+  const long pa = pix0 + li;
+  int n = 0, y = 0, x = 0;
+  const bool pvalid = pa < P;
+  if (pvalid) {
+    x = (int)(pa % a.W);
+    const long t = pa / a.W;
+    y = (int)(t % a.H);
+    n = (int)(t / a.H);
+  }
+  f4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  bool cvalid[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) cvalid[b] = co0 + b * 16 + li < a.Cout;
+  for (int tap = 0; tap < 9; tap++) {
+    int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    if (a.flip) {
+      dy = -dy;
+      dx = -dx;
+    }
+    const int yy = y + dy, xx = x + dx;
+    const bool ok = pvalid && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    const bf16* src = a.in + ((long)(n * a.H + (ok ? yy : 0)) * a.W + (ok ? xx : 0)) * a.Cin + 4 * lk;
+    const bf16* wt = a.w + (long)tap * a.Cout * a.Cin + 4 * lk;
+    for (int c0 = 0; c0 < a.Cin; c0 += 16) {
+      s4 av = s4{0, 0, 0, 0};
+      if (ok) av = *(const s4*)(src + c0);
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        s4 bv = s4{0, 0, 0, 0};
+        if (cvalid[b]) bv = *(const s4*)(wt + (long)(co0 + b * 16 + li) * a.Cin + c0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[b], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int co = co0 + b * 16 + li;
+    if (!cvalid[b]) continue;
+    const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const long p = pix0 + 4 * lk + v;
+      if (p >= P) continue;
+      float val = acc[b][v] + bb;
+      if (a.relu) val = val > 0.f ? val : 0.f;
+      if (a.out) a.out[p * a.Cout + co] = f2bf(val);
+      if (a.out32) a.out32[p * a.Cout + co] = val;
+    }
+  }
+}
+
+// One wavefront per (pixel range, tap, 16x16 tile of (co, ci)).
+__global__ void __launch_bounds__(64)
+    wgrad_mfma(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
+               int N, int H, int W, int Cin, int Cout, long chunk) {
+  const int lane = threadIdx.x;
+  const int li = lane & 15, lk = lane >> 4;
+  const int split = blockIdx.x, tap = blockIdx.y;
+  const int ncib = Cin / 16;
+  const int cob = blockIdx.z / ncib, cib = blockIdx.z % ncib;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const long P = (long)N * H * W;
+  const long p_begin = (long)split * chunk;
+  const long p_end = p_begin + chunk < P ? p_begin + chunk : P;
+  const int co = cob * 16 + li, ci = cib * 16 + li;
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  for (long p0 = p_begin; p0 < p_end; p0 += 16) {
+    short av[4], bv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const long p = p0 + 4 * lk + v;
+      av[v] = 0;
+      bv[v] = 0;
+      if (p < p_end) {
+        av[v] = (short)dz[p * Cout + co];
+        const int px = (int)(p % W);
+        const long t = p / W;
+        const int py = (int)(t % H);
+        const int pn = (int)(t / H);
+        const int yy = py + dy, xx = px + dx;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+          bv[v] = (short)x[((long)(pn * H + yy) * W + xx) * Cin + ci];
+      }
+    }
+    const s4 a4 = s4{av[0], av[1], av[2], av[3]};
+    const s4 b4 = s4{bv[0], bv[1], bv[2], bv[3]};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4, b4, acc, 0, 0, 0);
+  }
+  // D[i = co][j = ci]: lane holds co = cob*16 + 4*lk + v, ci = cib*16 + li
+  float* dst = part + ((long)split * 9 + tap) * Cout * Cin;
+#pragma unroll
+  for (int v = 0; v < 4; v++) dst[(long)(cob * 16 + 4 * lk + v) * Cin + cib * 16 + li] = acc[v];
+}
+
+__global__ void wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, long n,
+                             int nsplit) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int k = 0; k < nsplit; k++) s += part[(long)k * n + i];
+    dW[i] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256) bias_grad(const bf16* __restrict__ dz, float* __restrict__ db,
+                                                 long P, int C) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long p = threadIdx.x; p < P; p += blockDim.x) s += bf2f(dz[p * C + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = red[0];
+}
+
+__global__ void combine_kernel(bf16* __restrict__ out, int ost, int ooff, const bf16* __restrict__ a,
+                               int ast, int aoff, const bf16* __restrict__ b, int bst, int boff,
+                               const bf16* __restrict__ mask, int mst, int moff, int nc, long npix) {
+  const long total = npix * nc;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / nc;
+    const int c = (int)(i % nc);
+    float v = bf2f(a[p * ast + aoff + c]);
+    if (b) v += bf2f(b[p * bst + boff + c]);
+    if (mask && !(bf2f(mask[p * mst + moff + c]) > 0.f)) v = 0.f;
+    out[p * ost + ooff + c] = f2bf(v);
+  }
+}
+
+// h, w: sizes of the fine grid.
+__global__ void pool2_kernel(const bf16* __restrict__ in, bf16* __restrict__ out, int N, int h, int w,
+                             int C) {
+  const int h2 = h / 2, w2 = w / 2;
+  const long total = (long)N * h2 * w2 * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long t = i / C;
+    const int x = (int)(t % w2);
+    t /= w2;
+    const int y = (int)(t % h2);
+    const int n = (int)(t / h2);
+    const long b = ((long)(n * h + 2 * y) * w + 2 * x) * C + c;
+    const float s = (bf2f(in[b]) + bf2f(in[b + C])) + (bf2f(in[b + (long)w * C]) + bf2f(in[b + (long)w * C + C]));
+    out[i] = f2bf(0.25f * s);
+  }
+}
+
+__global__ void pool2_bwd_kernel(const bf16* __restrict__ dy, bf16* __restrict__ dx, int N, int h,
+                                 int w, int C) {
+  const long total = (long)N * h * w * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long t = i / C;
+    const int x = (int)(t % w);
+    t /= w;
+    const int y = (int)(t % h);
+    const int n = (int)(t / h);
+    dx[i] = f2bf(0.25f * bf2f(dy[((long)(n * (h / 2) + y / 2) * (w / 2) + x / 2) * C + c]));
+  }
+}
+
+__global__ void up2_kernel(const bf16* __restrict__ in, bf16* __restrict__ out, int N, int h, int w,
+                           int C) {
+  const long total = (long)N * h * w * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long t = i / C;
+    const int x = (int)(t % w);
+    t /= w;
+    const int y = (int)(t % h);
+    const int n = (int)(t / h);
+    out[i] = in[((long)(n * (h / 2) + y / 2) * (w / 2) + x / 2) * C + c];
+  }
+}
+
+__global__ void up2_bwd_kernel(const bf16* __restrict__ dy, bf16* __restrict__ dx, int N, int h,
+                               int w, int C) {
+  const int h2 = h / 2, w2 = w / 2;
+  const long total = (long)N * h2 * w2 * C;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    long t = i / C;
+    const int x = (int)(t % w2);
+    t /= w2;
+    const int y = (int)(t % h2);
+    const int n = (int)(t / h2);
+    const long b = ((long)(n * h + 2 * y) * w + 2 * x) * C + c;
+    const float s = (bf2f(dy[b]) + bf2f(dy[b + C])) + (bf2f(dy[b + (long)w * C]) + bf2f(dy[b + (long)w * C + C]));
+    dx[i] = f2bf(s);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    mse_kernel(const float* __restrict__ out32, int cst, const float* __restrict__ target, int nc,
+               long npix, bf16* __restrict__ dout, double* __restrict__ part) {
+  __shared__ double red[256];
+  const double scale = 2.0 / ((double)npix * nc);
+  double s = 0.0;
+  const long total = npix * cst;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / cst;
+    const int c = (int)(i % cst);
+    float g = 0.f;
+    if (c < nc) {
+      const double d = (double)out32[i] - (double)target[p * nc + c];
+      s += d * d;
+      g = (float)(scale * d);
+    }
+    dout[i] = f2bf(g);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int m = 128; m > 0; m >>= 1) {
+    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+__global__ void weights_bf16_kernel(const float* __restrict__ w32, bf16* __restrict__ wb,
+                                    bf16* __restrict__ wbt, int Cout, int Cin) {
+  const long total = 9L * Cout * Cin;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % Cin);
+    const long t = i / Cin;
+    const int co = (int)(t % Cout);
+    const int tap = (int)(t / Cout);
+    const bf16 v = f2bf(w32[i]);
+    wb[i] = v;
+    wbt[((long)tap * Cin + ci) * Cout + co] = v;
+  }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ in, bf16* __restrict__ out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = f2bf(in[i]);
+}
+
+__global__ void from_bf16_kernel(const bf16* __restrict__ in, float* __restrict__ out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = bf2f(in[i]);
+}
+
+template <typename T>
+__global__ void pack_kernel(const T* f0, const T* f1, const T* f2, const T* f3, const T* f4_,
+                            const T* f5, int dims0, int g0, int g1, int n0, int n1, double x0,
+                            double dx, double dy, double rpow, double extra, void* out, int out32,
+                            int cst, long base) {
+  const long total = (long)n0 * n1 * cst;
+  const T* F[6] = {f0, f1, f2, f3, f4_, f5};
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(t % cst);
+    const long pix = t / cst;
+    const int i = (int)(pix % n0);
+    const int j = (int)(pix / n0);
+    double v = 0.0;
+    if (c < 6) {
+      const double r = exp(x0 + dx * i);
+      const double th = dy * j;
+      const double h = (c % 3 == 2) ? r * sin(th) : r;
+      v = (double)F[c][(long)(i + g0) + (long)(j + g1) * dims0] * h * pow(r, rpow);
+    } else if (c == 6) {
+      v = extra;
+    }
+    const long o = base + t;
+    if (out32) ((float*)out)[o] = (float)v;
+    else ((bf16*)out)[o] = f2bf((float)v);
+  }
+}
+
+template <typename T>
+__global__ void unpack_kernel(const float* in, int cst, long base, T* f0, T* f1, T* f2, T* f3,
+                              T* f4_, T* f5, int dims0, int g0, int g1, int n0, int n1, double x0,
+                              double dx, double dy, double rpow) {
+  const long total = (long)n0 * n1 * 6;
+  T* F[6] = {f0, f1, f2, f3, f4_, f5};
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(t % 6);
+    const long pix = t / 6;
+    const int i = (int)(pix % n0);
+    const int j = (int)(pix / n0);
+    const double r = exp(x0 + dx * i);
+    const double th = dy * j;
+    const double h = (c % 3 == 2) ? r * sin(th) : r;
+    const double den = h * pow(r, rpow);
+    const double v = fabs(den) > 1e-12 ? (double)in[base + pix * cst + c] / den : 0.0;
+    F[c][(long)(i + g0) + (long)(j + g1) * dims0] = (T)v;
+  }
+}
+
+int grid_for(long total, int block = 256) {
+  long b = (total + block - 1) / block;
+  return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+}  // namespace
+}  // namespace coffee
+
+using namespace coffee;
+
+extern "C" {
+
+int coffee_unet_conv3x3(const void* in, const void* w, const float* bias, void* out, float* out32,
+                        int n, int h, int wd, int cin, int cout, int flip, int relu, void* stream) {
+  if (!in || !w || (!out && !out32) || n < 1 || h < 1 || wd < 1 || cin < 16 || cout < 16 ||
+      cin % 16 || cout % 16)
+    return fail(nullptr, COFFEE_ERR_INVALID, "conv3x3: bad arguments (channels must be multiples of 16)");
+  ConvArgs a{(const bf16*)in, (const bf16*)w, bias, (bf16*)out, out32, n, h, wd, cin, cout, flip, relu};
+  hipStream_t s = (hipStream_t)stream;
+  const long P = (long)n * h * wd;
+  const long bx = (P + 63) / 64;
+  if (bx > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3: too many pixels");
+  if (cout % 64 == 0) {
+    conv3x3_mfma<4><<<dim3((unsigned)bx, cout / 64), 256, 0, s>>>(a);
+  } else if (cout % 32 == 0) {
+    conv3x3_mfma<2><<<dim3((unsigned)bx, cout / 32), 256, 0, s>>>(a);
+  } else {
+    conv3x3_mfma<1><<<dim3((unsigned)bx, cout / 16), 256, 0, s>>>(a);
+  }
+  return launch_check("conv3x3");
+}
+
+int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int nsplit, float* dW,
+                              float* db, int n, int h, int wd, int cin, int cout, void* stream) {
+  if (!x || !dz || !partial || !dW || nsplit < 1 || cin % 16 || cout % 16 || cin < 16 || cout < 16)
+    return fail(nullptr, COFFEE_ERR_INVALID, "conv3x3_wgrad: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const long P = (long)n * h * wd;
+  long chunk = (P + nsplit - 1) / nsplit;
+  chunk = (chunk + 15) / 16 * 16;
+  const int used = (int)((P + chunk - 1) / chunk);
+  wgrad_mfma<<<dim3(used, 9, (cout / 16) * (cin / 16)), 64, 0, s>>>(
+      (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
+  int rc = launch_check("conv3x3_wgrad");
+  if (rc) return rc;
+  const long nw = 9L * cout * cin;
+  wgrad_reduce<<<grid_for(nw), 256, 0, s>>>(partial, dW, nw, used);
+  rc = launch_check("wgrad_reduce");
+  if (rc || !db) return rc;
+  bias_grad<<<cout, 256, 0, s>>>((const bf16*)dz, db, P, cout);
+  return launch_check("bias_grad");
+}
+
+int coffee_unet_combine(void* out, int ost, int ooff, const void* a, int ast, int aoff,
+                        const void* b, int bst, int boff, const void* mask, int mst, int moff,
+                        int nc, long npix, void* stream) {
+  if (!out || !a || nc < 1 || npix < 1 || ooff + nc > ost || aoff + nc > ast ||
+      (b && boff + nc > bst) || (mask && moff + nc > mst))
+    return fail(nullptr, COFFEE_ERR_INVALID, "combine: bad arguments");
+  combine_kernel<<<grid_for(npix * nc), 256, 0, (hipStream_t)stream>>>(
+      (bf16*)out, ost, ooff, (const bf16*)a, ast, aoff, (const bf16*)b, bst, boff,
+      (const bf16*)mask, mst, moff, nc, npix);
+  return launch_check("combine");
+}
+
+int coffee_unet_pool2(const void* in, void* out, int n, int h, int wd, int c, void* stream) {
+  if (!in || !out || h % 2 || wd % 2) return fail(nullptr, COFFEE_ERR_INVALID, "pool2: even sizes required");
+  pool2_kernel<<<grid_for((long)n * h * wd * c / 4), 256, 0, (hipStream_t)stream>>>(
+      (const bf16*)in, (bf16*)out, n, h, wd, c);
+  return launch_check("pool2");
+}
+
+int coffee_unet_pool2_bwd(const void* dy, void* dx, int n, int h, int wd, int c, void* stream) {
+  if (!dy || !dx || h % 2 || wd % 2) return fail(nullptr, COFFEE_ERR_INVALID, "pool2_bwd: even sizes required");
+  pool2_bwd_kernel<<<grid_for((long)n * h * wd * c), 256, 0, (hipStream_t)stream>>>(
+      (const bf16*)dy, (bf16*)dx, n, h, wd, c);
+  return launch_check("pool2_bwd");
+}
+
+int coffee_unet_up2(const void* in, void* out, int n, int h, int wd, int c, void* stream) {
+  if (!in || !out || h % 2 || wd % 2) return fail(nullptr, COFFEE_ERR_INVALID, "up2: even sizes required");
+  up2_kernel<<<grid_for((long)n * h * wd * c), 256, 0, (hipStream_t)stream>>>(
+      (const bf16*)in, (bf16*)out, n, h, wd, c);
+  return launch_check("up2");
+}
+
+int coffee_unet_up2_bwd(const void* dy, void* dx, int n, int h, int wd, int c, void* stream) {
+  if (!dy || !dx || h % 2 || wd % 2) return fail(nullptr, COFFEE_ERR_INVALID, "up2_bwd: even sizes required");
+  up2_bwd_kernel<<<grid_for((long)n * h * wd * c / 4), 256, 0, (hipStream_t)stream>>>(
+      (const bf16*)dy, (bf16*)dx, n, h, wd, c);
+  return launch_check("up2_bwd");
+}
+
+int coffee_unet_mse(const float* out32, int cst, const float* target, int nc, long npix,
+                    void* dout, double* loss, void* stream) {
+  if (!out32 || !target || !dout || !loss || nc < 1 || nc > cst)
+    return fail(nullptr, COFFEE_ERR_INVALID, "mse: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = 1024;
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * blocks) != hipSuccess)
+    return fail(nullptr, COFFEE_ERR_HIP, "mse: hipMalloc failed");
+  mse_kernel<<<blocks, 256, 0, s>>>(out32, cst, target, nc, npix, (bf16*)dout, part);
+  hipError_t e = hipGetLastError();
+  double h[1024];
+  if (e == hipSuccess) e = hipMemcpyAsync(h, part, sizeof(double) * blocks, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, std::string("mse: ") + hipGetErrorString(e));
+  double tot = 0.0;
+  for (int b = 0; b < blocks; b++) tot += h[b];
+  *loss = tot / ((double)npix * nc);
+  return COFFEE_OK;
+}
+
+int coffee_unet_adam(float* p, const float* g, float* m, float* v, long n, float lr, float beta1,
+                     float beta2, float eps, int t, void* wb, void* wbt, int cout, int cin,
+                     void* stream) {
+  if (!p || !g || !m || !v || n < 1 || t < 1) return fail(nullptr, COFFEE_ERR_INVALID, "adam: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
+  adam_kernel<<<grid_for(n), 256, 0, s>>>(p, g, m, v, n, lr, beta1, beta2, eps, c1, c2);
+  int rc = launch_check("adam");
+  if (rc || !wb) return rc;
+  if (!wbt || 9L * cout * cin != n) return fail(nullptr, COFFEE_ERR_INVALID, "adam: weight shape mismatch");
+  weights_bf16_kernel<<<grid_for(n), 256, 0, s>>>(p, (bf16*)wb, (bf16*)wbt, cout, cin);
+  return launch_check("weights_bf16");
+}
+
+int coffee_unet_weights_bf16(const float* w32, void* wb, void* wbt, int cout, int cin, void* stream) {
+  if (!w32 || !wb || !wbt) return fail(nullptr, COFFEE_ERR_INVALID, "weights_bf16: null argument");
+  weights_bf16_kernel<<<grid_for(9L * cout * cin), 256, 0, (hipStream_t)stream>>>(
+      w32, (bf16*)wb, (bf16*)wbt, cout, cin);
+  return launch_check("weights_bf16");
+}
+
+int coffee_unet_to_bf16(const float* in, void* out, long n, void* stream) {
+  if (!in || !out) return fail(nullptr, COFFEE_ERR_INVALID, "to_bf16: null argument");
+  to_bf16_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(in, (bf16*)out, n);
+  return launch_check("to_bf16");
+}
+
+int coffee_unet_from_bf16(const void* in, float* out, long n, void* stream) {
+  if (!in || !out) return fail(nullptr, COFFEE_ERR_INVALID, "from_bf16: null argument");
+  from_bf16_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>((const bf16*)in, out, n);
+  return launch_check("from_bf16");
+}
+
+int coffee_unet_pack(void* const fields[6], int scalar_bytes, int dims0, int g0, int g1, int n0,
+                     int n1, double x0, double dx, double dy, double rpow, double extra_channel,
+                     void* out, int out_fp32, int cst, int image, void* stream) {
+  if (!fields || !out || cst < 6 || n0 < 1 || n1 < 1 || image < 0)
+    return fail(nullptr, COFFEE_ERR_INVALID, "pack: bad arguments");
+  for (int q = 0; q < 6; q++)
+    if (!fields[q]) return fail(nullptr, COFFEE_ERR_INVALID, "pack: null field");
+  const long base = (long)image * n0 * n1 * cst;
+  const int blocks = grid_for((long)n0 * n1 * cst);
+  hipStream_t s = (hipStream_t)stream;
+  if (scalar_bytes == 4) {
+    float* const* f = (float* const*)fields;
+    pack_kernel<float><<<blocks, 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], f[5], dims0, g0, g1, n0,
+                                              n1, x0, dx, dy, rpow, extra_channel, out, out_fp32,
+                                              cst, base);
+  } else {
+    double* const* f = (double* const*)fields;
+    pack_kernel<double><<<blocks, 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], f[5], dims0, g0, g1,
+                                               n0, n1, x0, dx, dy, rpow, extra_channel, out,
+                                               out_fp32, cst, base);
+  }
+  return launch_check("pack");
+}
+
+int coffee_unet_unpack(const float* in, int cst, int image, void* const fields[6], int scalar_bytes,
+                       int dims0, int g0, int g1, int n0, int n1, double x0, double dx, double dy,
+                       double rpow, void* stream) {
+  if (!in || !fields || cst < 6 || n0 < 1 || n1 < 1 || image < 0)
+    return fail(nullptr, COFFEE_ERR_INVALID, "unpack: bad arguments");
+  for (int q = 0; q < 6; q++)
+    if (!fields[q]) return fail(nullptr, COFFEE_ERR_INVALID, "unpack: null field");
+  const long base = (long)image * n0 * n1 * cst;
+  const int blocks = grid_for((long)n0 * n1 * 6);
+  hipStream_t s = (hipStream_t)stream;
+  if (scalar_bytes == 4) {
+    float* const* f = (float* const*)fields;
+    unpack_kernel<float><<<blocks, 256, 0, s>>>(in, cst, base, f[0], f[1], f[2], f[3], f[4], f[5],
+                                                dims0, g0, g1, n0, n1, x0, dx, dy, rpow);
+  } else {
+    double* const* f = (double* const*)fields;
+    unpack_kernel<double><<<blocks, 256, 0, s>>>(in, cst, base, f[0], f[1], f[2], f[3], f[4], f[5],
+                                                 dims0, g0, g1, n0, n1, x0, dx, dy, rpow);
+  }
+  return launch_check("unpack");
+}
+
+}  // extern "C"
