@@ -17,6 +17,7 @@
 #include "coffee_hip.h"
 #include "ctx.hpp"
 #include "ez_gr.hpp"
+#include "ez_gr_tiled.hpp"
 
 namespace coffee {
 
@@ -109,6 +110,20 @@ void launch_gr_stage(bool dg, dim3 grid, dim3 block, hipStream_t s, const GrK<T>
   else gr_stage_kernel<T, F, L, false><<<grid, block, 0, s>>>(k, io, as, bs);
 }
 
+template <typename T, int TY, bool F, bool L>
+void launch_gr_tiled(bool dg, int blocks, hipStream_t s, const GrK<T>& k, const GrIO<T>& io, T as,
+                     T bs, const TileMap& tm) {
+  if (dg) gr_stage_tiled<T, TY, F, L, true><<<blocks, 64 * TY, 0, s>>>(k, io, as, bs, tm);
+  else gr_stage_tiled<T, TY, F, L, false><<<blocks, 64 * TY, 0, s>>>(k, io, as, bs, tm);
+}
+
+// Rows per tile of the tiled stage kernel: 8 in FP32 (512 threads), 4 in FP64
+// (256 threads; the register queues double in size).
+template <typename T>
+constexpr int gr_tile_rows() {
+  return sizeof(T) == 4 ? 8 : 4;
+}
+
 template <typename T>
 int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
   GrState* S = gr_state(c);
@@ -157,6 +172,21 @@ int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
   const int pblocks = std::min((total + 255) / 256, 8192);
   dim3 block(64, 4, 1);
   dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
+  // variant 0: tiled stage kernel; variant 1: point sweep + stage sweep
+  const bool tiled = c->variant != 1;
+  constexpr int TY = gr_tile_rows<T>();
+  TileMap tm;
+  tm.ntx = (k.n[0] + 63) / 64;
+  tm.nty = (k.n[1] + TY - 1) / TY;
+  {
+    // two workgroups per CU (256 CUs), whole columns when the plane has enough tiles
+    const int tiles = tm.ntx * tm.nty;
+    const int nzc = std::max(1, (512 + tiles - 1) / tiles);
+    tm.zc = c->zchunk > 0 ? c->zchunk : std::max(4, (k.n[2] + nzc - 1) / nzc);
+    if (tm.zc > k.n[2]) tm.zc = k.n[2];
+    tm.nzc = (k.n[2] + tm.zc - 1) / tm.zc;
+  }
+  const int tblocks = tm.ntx * tm.nty * tm.nzc;
   for (int st = 0; st < 5; st++) {
     T* const* in = (st % 2 == 0) ? A : B;
     T* const* out = (st % 2 == 0) ? B : A;
@@ -164,15 +194,23 @@ int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
       io.in[q] = in[q];
       io.out[q] = out[q];
     }
-    rc = timed(c, "gr_point", s, [&] { gr_point_kernel<T><<<pblocks, 256, 0, s>>>(k, io); });
-    if (rc) return rc;
     const T as = (T)kAs[st], bs = (T)kBs[st];
     const char* name = st == 0 ? "gr_stage_first" : (st == 4 ? "gr_stage_last" : "gr_stage");
-    rc = timed(c, name, s, [&] {
-      if (st == 0) launch_gr_stage<T, true, false>(dg, grid, block, s, k, io, as, bs);
-      else if (st == 4) launch_gr_stage<T, false, true>(dg, grid, block, s, k, io, as, bs);
-      else launch_gr_stage<T, false, false>(dg, grid, block, s, k, io, as, bs);
-    });
+    if (tiled) {
+      rc = timed(c, name, s, [&] {
+        if (st == 0) launch_gr_tiled<T, TY, true, false>(dg, tblocks, s, k, io, as, bs, tm);
+        else if (st == 4) launch_gr_tiled<T, TY, false, true>(dg, tblocks, s, k, io, as, bs, tm);
+        else launch_gr_tiled<T, TY, false, false>(dg, tblocks, s, k, io, as, bs, tm);
+      });
+    } else {
+      rc = timed(c, "gr_point", s, [&] { gr_point_kernel<T><<<pblocks, 256, 0, s>>>(k, io); });
+      if (rc) return rc;
+      rc = timed(c, name, s, [&] {
+        if (st == 0) launch_gr_stage<T, true, false>(dg, grid, block, s, k, io, as, bs);
+        else if (st == 4) launch_gr_stage<T, false, true>(dg, grid, block, s, k, io, as, bs);
+        else launch_gr_stage<T, false, false>(dg, grid, block, s, k, io, as, bs);
+      });
+    }
     if (rc) return rc;
     rc = gr_guards<T>(c, s, out, out, true);
     if (rc) return rc;

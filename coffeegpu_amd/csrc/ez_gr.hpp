@@ -261,31 +261,33 @@ __device__ __forceinline__ T gr_d1(const T* __restrict__ f, int c, int s, T c12)
   return fma(T(8), f[c + s] - f[c - s], f[c - 2 * s] - f[c + 2 * s]) * c12;
 }
 
-// One LSRK stage at interior cell (i, j, kk) (rk_step1 .cu:188-373, rk_step2
-// .cu:375-438, projections .cu:440-535, absorbing in the last stage).
+// Derivatives a stage needs at one cell, each a 4th-order centred difference
+// times inv_delta: ddx = d/dx of (E_y, E_z, H_y, H_z, sqrt(g) D^x, sqrt(g) B^x, P),
+// ddy = d/dy of (E_x, E_z, H_x, H_z, sqrt(g) D^y, sqrt(g) B^y, P),
+// ddz = d/dz of (E_x, E_y, H_x, H_y, sqrt(g) D^z, sqrt(g) B^z, P).
+// Which W array feeds each slot, per axis:
+constexpr int kGrDx[6] = {1, 2, 4, 5, 6, 9};
+constexpr int kGrDy[6] = {0, 2, 3, 5, 7, 10};
+constexpr int kGrDz[6] = {0, 1, 3, 4, 8, 11};
+
+// One LSRK stage at an interior cell from its derivatives (rk_step1
+// .cu:188-373, rk_step2 .cu:375-438, projections .cu:440-535, absorbing in
+// the last stage).  u: D, B, P at the cell (in), dprev: dU before the stage,
+// Ec: E_i at the cell.
 template <typename T, bool FIRST, bool LAST, bool DIAG>
-__device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io, int i, int j,
-                                              int kk, int c, T As, T Bs) {
-  const int sy = k.sy, sz = k.sz;
-  const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
-  const CksT<T> m = cks_scalar<T>(cks_point(k.a, x, y, z));
+__device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io, int c, double x,
+                                              double y, double z, const CksT<T>& m,
+                                              const T ddx[7], const T ddy[7], const T ddz[7],
+                                              const T Ec[3], T u[7], const T dprev[7], T As, T Bs) {
   const T tiny = TinyOf<T>::v;
-  const T* const* W = (const T* const*)io.W;
-  // curl numerators of E_i, H_i; rot = curl / sqrt(g) = alpha * curl
-  const T rotE[3] = {m.alpha * (gr_d1(W[2], c, sy, k.c12[1]) - gr_d1(W[1], c, sz, k.c12[2])),
-                     m.alpha * (gr_d1(W[0], c, sz, k.c12[2]) - gr_d1(W[2], c, 1, k.c12[0])),
-                     m.alpha * (gr_d1(W[1], c, 1, k.c12[0]) - gr_d1(W[0], c, sy, k.c12[1]))};
-  const T rotH[3] = {m.alpha * (gr_d1(W[5], c, sy, k.c12[1]) - gr_d1(W[4], c, sz, k.c12[2])),
-                     m.alpha * (gr_d1(W[3], c, sz, k.c12[2]) - gr_d1(W[5], c, 1, k.c12[0])),
-                     m.alpha * (gr_d1(W[4], c, 1, k.c12[0]) - gr_d1(W[3], c, sy, k.c12[1]))};
+  // rot = curl / sqrt(g) = alpha * curl
+  const T rotE[3] = {m.alpha * (ddy[1] - ddz[1]), m.alpha * (ddz[0] - ddx[1]),
+                     m.alpha * (ddx[0] - ddy[0])};
+  const T rotH[3] = {m.alpha * (ddy[3] - ddz[3]), m.alpha * (ddz[2] - ddx[3]),
+                     m.alpha * (ddx[2] - ddy[2])};
   // div4 (.cu:44-88): d_i(sqrt(g) f^i) / sqrt(g)
-  const T divD = m.alpha * (gr_d1(W[6], c, 1, k.c12[0]) + gr_d1(W[7], c, sy, k.c12[1]) +
-                            gr_d1(W[8], c, sz, k.c12[2]));
-  const T divB = m.alpha * (gr_d1(W[9], c, 1, k.c12[0]) + gr_d1(W[10], c, sy, k.c12[1]) +
-                            gr_d1(W[11], c, sz, k.c12[2]));
-  T u[7];
-#pragma unroll
-  for (int f = 0; f < 7; f++) u[f] = io.in[f][c];
+  const T divD = m.alpha * (ddx[4] + ddy[4] + ddz[4]);
+  const T divB = m.alpha * (ddx[5] + ddy[5] + ddz[5]);
   const T D[3] = {u[0], u[1], u[2]};
   const T B[3] = {u[3], u[4], u[5]};
   T Bd[3], Dd[3];
@@ -296,20 +298,18 @@ __device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io
     T B2 = dot3(B, Bd);
     if (B2 < tiny) B2 = tiny;
     const T Jp = dot3(Bd, rotH) - dot3(Dd, rotE);
-    const T E[3] = {W[0][c], W[1][c], W[2][c]};
     const T rho = divD * m.alpha;
-    J[0] = (rho * (E[1] * Bd[2] - E[2] * Bd[1]) + Jp * B[0]) / B2;
-    J[1] = (rho * (E[2] * Bd[0] - E[0] * Bd[2]) + Jp * B[1]) / B2;
-    J[2] = (rho * (E[0] * Bd[1] - E[1] * Bd[0]) + Jp * B[2]) / B2;
+    J[0] = (rho * (Ec[1] * Bd[2] - Ec[2] * Bd[1]) + Jp * B[0]) / B2;
+    J[1] = (rho * (Ec[2] * Bd[0] - Ec[0] * Bd[2]) + Jp * B[1]) / B2;
+    J[2] = (rho * (Ec[0] * Bd[1] - Ec[1] * Bd[0]) + Jp * B[2]) / B2;
   }
-  const T* P = io.in[6];
-  const T gP[3] = {gr_d1(P, c, 1, k.c12[0]), gr_d1(P, c, sy, k.c12[1]), gr_d1(P, c, sz, k.c12[2])};
+  const T gP[3] = {ddx[6], ddy[6], ddz[6]};
   const T lgP = dot3(m.l, gP);
   const T beta[3] = {m.bfac * m.l[0], m.bfac * m.l[1], m.bfac * m.l[2]};
   const T shape = T(0.5) * (T(1) - tanh((T)(m.r - 5.0)));
   T du[7];
 #pragma unroll
-  for (int f = 0; f < 7; f++) du[f] = FIRST ? T(0) : io.dU[f][c];
+  for (int f = 0; f < 7; f++) du[f] = FIRST ? T(0) : dprev[f];
   const T dt = k.dt;
 #pragma unroll
   for (int q = 0; q < 3; q++) {
@@ -343,6 +343,36 @@ __device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io
     io.divE[c] = divD;
     io.divB[c] = divB;
   }
+}
+
+// The stage at interior cell (i, j, kk) with E_i, H_i, sqrt(g) D, sqrt(g) B
+// read from the point sweep's arrays W.
+template <typename T, bool FIRST, bool LAST, bool DIAG>
+__device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io, int i, int j,
+                                              int kk, int c, T As, T Bs) {
+  const int sy = k.sy, sz = k.sz;
+  const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
+  const CksT<T> m = cks_scalar<T>(cks_point(k.a, x, y, z));
+  const T* const* W = (const T* const*)io.W;
+  const T* P = io.in[6];
+  T ddx[7], ddy[7], ddz[7];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    ddx[q] = gr_d1(W[kGrDx[q]], c, 1, k.c12[0]);
+    ddy[q] = gr_d1(W[kGrDy[q]], c, sy, k.c12[1]);
+    ddz[q] = gr_d1(W[kGrDz[q]], c, sz, k.c12[2]);
+  }
+  ddx[6] = gr_d1(P, c, 1, k.c12[0]);
+  ddy[6] = gr_d1(P, c, sy, k.c12[1]);
+  ddz[6] = gr_d1(P, c, sz, k.c12[2]);
+  const T Ec[3] = {W[0][c], W[1][c], W[2][c]};
+  T u[7], dp[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) {
+    u[f] = io.in[f][c];
+    dp[f] = FIRST ? T(0) : io.dU[f][c];
+  }
+  gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, x, y, z, m, ddx, ddy, ddz, Ec, u, dp, As, Bs);
 }
 
 template <typename T, bool FIRST, bool LAST, bool DIAG>

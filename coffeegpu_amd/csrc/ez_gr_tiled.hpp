@@ -1,0 +1,236 @@
+// LDS-tiled, z-marching stage kernel of the Kerr-Schild GR path (gfx950).
+//
+// One launch per RK stage replaces the point sweep + stage sweep of the
+// simple schedule: a 64 x TY workgroup marches its tile column through a
+// chunk of z planes and computes E_i, H_i, sqrt(g) D, sqrt(g) B, P at every
+// point it needs on the fly (FP64 metric per point), so the only HBM traffic
+// is the state U, the LSRK register dU and their updates.
+//   x-y derivatives read 11 per-point values staged in double-buffered LDS
+//   (tile + 2-cell halo, one barrier per plane);
+//   z derivatives read a register queue of the 7 z-differenced values of the
+//   planes z-2 .. z+2; the raw state and metric of planes z .. z+2 wait in
+//   registers, plane z+3 is loaded one iteration ahead.
+// Each point's values come from the same functions as the simple schedule
+// (gr_point_values, cks_point) and the cell update is the same gr_stage_core,
+// so both schedules give bitwise the same results.
+#pragma once
+
+#include "ez_flat_tiled.hpp"
+#include "ez_gr.hpp"
+
+namespace coffee {
+
+template <typename T>
+__device__ __forceinline__ CksT<T> gr_metric_at(const GrK<T>& k, int ai, int aj, int ak) {
+  return cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak)));
+}
+
+// LDS slots: 0-2 E_i, 3-5 H_i, 6 sqrt(g) D^x, 7 sqrt(g) D^y, 8 sqrt(g) B^x,
+// 9 sqrt(g) B^y, 10 P.
+template <typename T>
+__device__ __forceinline__ void gr_point_full(const CksT<T>& m, const T u[7], T v[11]) {
+  const T D[3] = {u[0], u[1], u[2]};
+  const T B[3] = {u[3], u[4], u[5]};
+  T E[3], H[3];
+  gr_point_values(m, D, B, E, H);
+  v[0] = E[0];
+  v[1] = E[1];
+  v[2] = E[2];
+  v[3] = H[0];
+  v[4] = H[1];
+  v[5] = H[2];
+  v[6] = m.sg * D[0];
+  v[7] = m.sg * D[1];
+  v[8] = m.sg * B[0];
+  v[9] = m.sg * B[1];
+  v[10] = u[6];
+}
+
+// The values differenced along z, in the order of gr_stage_core's ddz.
+template <typename T>
+__device__ __forceinline__ void gr_point_z(const CksT<T>& m, const T u[7], T v[7]) {
+  const T D[3] = {u[0], u[1], u[2]};
+  const T B[3] = {u[3], u[4], u[5]};
+  T E[3], H[3];
+  gr_point_values(m, D, B, E, H);
+  v[0] = E[0];
+  v[1] = E[1];
+  v[2] = H[0];
+  v[3] = H[1];
+  v[4] = m.sg * D[2];
+  v[5] = m.sg * B[2];
+  v[6] = u[6];
+}
+
+template <typename T, int TY, bool FIRST, bool LAST, bool DIAG>
+__global__ void __launch_bounds__(64 * TY)
+    gr_stage_tiled(const GrK<T> k, const GrIO<T> io, const T As, const T Bs, const TileMap tm) {
+  constexpr int TX = 64, R = 2, LX = TX + 2 * R, LY = TY + 2 * R, NL = 11, NT = TX * TY;
+  constexpr int H = 2 * R * (TX + TY);
+  constexpr int HPT = (H + NT - 1) / NT;
+  __shared__ T lds[2][NL][LY * LX];
+  // LDS slots differenced along x and along y (the order of ddx / ddy)
+  constexpr int SX[7] = {1, 2, 4, 5, 6, 8, 10};
+  constexpr int SY[7] = {0, 2, 3, 5, 7, 9, 10};
+
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = L % tm.ntx;
+  const int rest = L / tm.ntx;
+  const int ty = rest % tm.nty;
+  const int tz = rest / tm.nty;
+  const int i0 = tx * TX, j0 = ty * TY, z0 = tz * tm.zc;
+  const int z1 = min(z0 + tm.zc, k.n[2]);
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = t / TX;
+  const int gi = i0 + lx, gj = j0 + ly;
+  const bool active = gi < k.n[0] && gj < k.n[1];
+  // guards are filled, so no axis wraps; clamping keeps every access inside
+  // the guarded array (guard >= 3 is enforced at context creation)
+  const int ai = amap(gi, k.n[0], k.guard[0], 0);
+  const int aj = amap(gj, k.n[1], k.guard[1], 0);
+  const int col = ai + aj * k.sy;
+  const int slot = (ly + R) * LX + (lx + R);
+
+  int hcol[HPT], hslot[HPT], hai[HPT], haj[HPT];
+  bool hok[HPT];
+#pragma unroll
+  for (int h = 0; h < HPT; h++) {
+    const int idx = t + h * NT;
+    hok[h] = idx < H;
+    int hx = 0, hy = 0;
+    if (hok[h]) halo_xy<TX, TY, R>(idx, hx, hy);
+    hai[h] = amap(i0 + hx, k.n[0], k.guard[0], 0);
+    haj[h] = amap(j0 + hy, k.n[1], k.guard[1], 0);
+    hcol[h] = hai[h] + haj[h] * k.sy;
+    hslot[h] = (hy + R) * LX + (hx + R);
+  }
+
+  T qz[5][7];     // z-differenced values of planes z-2 .. z+2
+  T rawq[3][7];   // state of planes z .. z+2 at this column
+  CksT<T> mq[3];  // metric of planes z .. z+2 at this column
+  T inc[7];       // state of plane z+3 (in flight)
+#pragma unroll
+  for (int d = 0; d < 5; d++) {
+    const int ak = amap(z0 - 2 + d, k.n[2], k.guard[2], 0);
+    T u[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) u[f] = io.in[f][col + ak * k.sz];
+    const CksT<T> m = gr_metric_at(k, ai, aj, ak);
+    gr_point_z(m, u, qz[d]);
+    if (d >= 2) {
+#pragma unroll
+      for (int f = 0; f < 7; f++) rawq[d - 2][f] = u[f];
+      mq[d - 2] = m;
+    }
+  }
+  {
+    const int ak = amap(z0 + 3, k.n[2], k.guard[2], 0);
+#pragma unroll
+    for (int f = 0; f < 7; f++) inc[f] = io.in[f][col + ak * k.sz];
+  }
+  T hv[HPT][7];
+  {
+    const int ak = amap(z0, k.n[2], k.guard[2], 0);
+#pragma unroll
+    for (int h = 0; h < HPT; h++)
+      if (hok[h])
+#pragma unroll
+        for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + ak * k.sz];
+  }
+  T dp[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) dp[f] = T(0);
+  if (!FIRST && active) {
+    const int c0 = col + (z0 + k.guard[2]) * k.sz;
+#pragma unroll
+    for (int f = 0; f < 7; f++) dp[f] = io.dU[f][c0];
+  }
+
+  for (int z = z0; z < z1; z++) {
+    const int buf = (z - z0) & 1;
+    const int akz = z + k.guard[2];
+    T nxt[7];
+    {
+      const int ak = amap(z + 4, k.n[2], k.guard[2], 0);
+#pragma unroll
+      for (int f = 0; f < 7; f++) nxt[f] = io.in[f][col + ak * k.sz];
+    }
+    T Ec[3];
+    {
+      T v[11];
+      gr_point_full(mq[0], rawq[0], v);
+#pragma unroll
+      for (int f = 0; f < NL; f++) lds[buf][f][slot] = v[f];
+      Ec[0] = v[0];
+      Ec[1] = v[1];
+      Ec[2] = v[2];
+    }
+#pragma unroll
+    for (int h = 0; h < HPT; h++)
+      if (hok[h]) {
+        const CksT<T> m = gr_metric_at(k, hai[h], haj[h], akz);
+        T v[11];
+        gr_point_full(m, hv[h], v);
+#pragma unroll
+        for (int f = 0; f < NL; f++) lds[buf][f][hslot[h]] = v[f];
+      }
+    if (z + 1 < z1) {
+      const int off = (akz + 1) * k.sz;
+#pragma unroll
+      for (int h = 0; h < HPT; h++)
+        if (hok[h])
+#pragma unroll
+          for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + off];
+    }
+    T dn[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) dn[f] = T(0);
+    if (!FIRST && active && z + 1 < z1) {
+      const int c1 = col + (akz + 1) * k.sz;
+#pragma unroll
+      for (int f = 0; f < 7; f++) dn[f] = io.dU[f][c1];
+    }
+    __syncthreads();
+    if (active) {
+      T ddx[7], ddy[7], ddz[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        const T* px = &lds[buf][SX[q]][slot];
+        ddx[q] = fma(T(8), px[1] - px[-1], px[-2] - px[2]) * k.c12[0];
+        const T* py = &lds[buf][SY[q]][slot];
+        ddy[q] = fma(T(8), py[LX] - py[-LX], py[-2 * LX] - py[2 * LX]) * k.c12[1];
+        ddz[q] = fma(T(8), qz[3][q] - qz[1][q], qz[0][q] - qz[4][q]) * k.c12[2];
+      }
+      T u[7];
+#pragma unroll
+      for (int f = 0; f < 7; f++) u[f] = rawq[0][f];
+      const int c = col + akz * k.sz;
+      gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, gr_pos(k, 0, ai), gr_pos(k, 1, aj),
+                                          gr_pos(k, 2, akz), mq[0], ddx, ddy, ddz, Ec, u, dp, As,
+                                          Bs);
+    }
+    // advance the queues: plane z+3 enters
+    {
+      const int ak3 = amap(z + 3, k.n[2], k.guard[2], 0);
+      const CksT<T> m3 = gr_metric_at(k, ai, aj, ak3);
+#pragma unroll
+      for (int d = 0; d < 4; d++)
+#pragma unroll
+        for (int f = 0; f < 7; f++) qz[d][f] = qz[d + 1][f];
+      gr_point_z(m3, inc, qz[4]);
+#pragma unroll
+      for (int f = 0; f < 7; f++) {
+        rawq[0][f] = rawq[1][f];
+        rawq[1][f] = rawq[2][f];
+        rawq[2][f] = inc[f];
+        inc[f] = nxt[f];
+        dp[f] = dn[f];
+      }
+      mq[0] = mq[1];
+      mq[1] = mq[2];
+      mq[2] = m3;
+    }
+  }
+}
+
+}  // namespace coffee

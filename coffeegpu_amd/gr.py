@@ -46,4 +46,7 @@ class FieldSolverGREZ(FieldSolverEZ):
         self._diag = False
 
     def set_schedule(self, variant: int = 0, zchunk: int = 0):
-        raise RuntimeError("the GR solver has a single schedule")
+        """0 = LDS-tiled stage kernel computing E, H on the fly (default);
+        1 = point sweep (E, H arrays) + stage sweep.  Both give bitwise the
+        same results."""
+        _lib.check(self._lib.coffee_set_schedule(self._ctx, int(variant), int(zchunk)), self._ctx)
