@@ -265,8 +265,9 @@ def run_gr(args):
         arr.set(f[name])
     del f
     solver = FieldSolverGREZ(data, env)
-    print(f"[bench] gr-wald {n}^3 a={sp.a} set-up {time.perf_counter() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    solver.set_schedule(args.variant, args.zchunk)
+    print(f"[bench] gr-wald {n}^3 a={sp.a} variant {args.variant} set-up "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         solver.evolve_fields(0.0)
     torch.cuda.synchronize()
@@ -296,7 +297,8 @@ def run_gr(args):
         "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
         "config": {"workload": f"gr-wald {n}^3 box [-20,20]^3 a={sp.a}, pml 32 (sigpml 10, "
                                "pmllen 1), horizon excision, FP64 metric on the fly",
-                   "cells": cells, "parallelism": "single GPU"},
+                   "cells": cells, "schedule": "tiled" if args.variant != 1 else "point+stage",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gr_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},

@@ -219,7 +219,10 @@ int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
     io.in[q] = B[q];
     io.out[q] = A[q];
   }
-  rc = timed(c, "gr_ko", s, [&] { gr_ko_kernel<T><<<grid, block, 0, s>>>(k, io); });
+  rc = timed(c, "gr_ko", s, [&] {
+    if (tiled) gr_ko_tiled<T, TY><<<tblocks, 64 * TY, 0, s>>>(k, io, tm);
+    else gr_ko_kernel<T><<<grid, block, 0, s>>>(k, io);
+  });
   if (rc) return rc;
   return gr_guards<T>(c, s, A, A, true);
 }

@@ -233,4 +233,106 @@ __global__ void __launch_bounds__(64 * TY)
   }
 }
 
+// KO on D, B, P (.cu:537-608) + the metric projections, tiled like the flat
+// KO kernel: the 7 fields of the tile + 3-cell halo in double-buffered LDS,
+// z neighbours in a register queue.  Same arithmetic as gr_ko_kernel.
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    gr_ko_tiled(const GrK<T> k, const GrIO<T> io, const TileMap tm) {
+  constexpr int TX = 64, R = 3, LX = TX + 2 * R, LY = TY + 2 * R, NF = 7, NT = TX * TY;
+  constexpr int H = 2 * R * (TX + TY);
+  constexpr int HPT = (H + NT - 1) / NT;
+  constexpr int Q = 2 * R + 1;
+  __shared__ T lds[2][NF][LY * LX];
+
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = L % tm.ntx;
+  const int rest = L / tm.ntx;
+  const int ty = rest % tm.nty;
+  const int tz = rest / tm.nty;
+  const int i0 = tx * TX, j0 = ty * TY, z0 = tz * tm.zc;
+  const int z1 = min(z0 + tm.zc, k.n[2]);
+  const int t = threadIdx.x;
+  const int lx = t % TX, ly = t / TX;
+  const int gi = i0 + lx, gj = j0 + ly;
+  const bool active = gi < k.n[0] && gj < k.n[1];
+  const int ai = amap(gi, k.n[0], k.guard[0], 0);
+  const int aj = amap(gj, k.n[1], k.guard[1], 0);
+  const int col = ai + aj * k.sy;
+  const int slot = (ly + R) * LX + (lx + R);
+
+  int hcol[HPT], hslot[HPT];
+  bool hok[HPT];
+#pragma unroll
+  for (int h = 0; h < HPT; h++) {
+    const int idx = t + h * NT;
+    hok[h] = idx < H;
+    int hx = 0, hy = 0;
+    if (hok[h]) halo_xy<TX, TY, R>(idx, hx, hy);
+    hcol[h] = amap(i0 + hx, k.n[0], k.guard[0], 0) + amap(j0 + hy, k.n[1], k.guard[1], 0) * k.sy;
+    hslot[h] = (hy + R) * LX + (hx + R);
+  }
+  T q[Q + 1][NF];
+#pragma unroll
+  for (int d = 0; d < Q; d++) {
+    const int off = amap(z0 - R + d, k.n[2], k.guard[2], 0) * k.sz;
+#pragma unroll
+    for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
+  }
+  T hv[HPT][NF];
+  {
+    const int off = amap(z0, k.n[2], k.guard[2], 0) * k.sz;
+#pragma unroll
+    for (int h = 0; h < HPT; h++)
+      if (hok[h])
+#pragma unroll
+        for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+  }
+  for (int z = z0; z < z1; z++) {
+    const int buf = (z - z0) & 1;
+    {
+      const int off = amap(z + R + 1, k.n[2], k.guard[2], 0) * k.sz;
+#pragma unroll
+      for (int f = 0; f < NF; f++) q[Q][f] = io.in[f][col + off];
+    }
+    const int az = z + k.guard[2];
+#pragma unroll
+    for (int f = 0; f < NF; f++) lds[buf][f][slot] = q[R][f];
+#pragma unroll
+    for (int h = 0; h < HPT; h++)
+      if (hok[h])
+#pragma unroll
+        for (int f = 0; f < NF; f++) lds[buf][f][hslot[h]] = hv[h][f];
+    if (z + 1 < z1) {
+      const int off = (az + 1) * k.sz;
+#pragma unroll
+      for (int h = 0; h < HPT; h++)
+        if (hok[h])
+#pragma unroll
+          for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+    }
+    __syncthreads();
+    if (active) {
+      T u[7];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const T* p = &lds[buf][f][slot];
+        const T kx = d6v(p[-3], p[-2], p[-1], p[0], p[1], p[2], p[3]);
+        const T ky = d6v(p[-3 * LX], p[-2 * LX], p[-LX], p[0], p[LX], p[2 * LX], p[3 * LX]);
+        const T kz = d6v(q[0][f], q[1][f], q[2][f], q[3][f], q[4][f], q[5][f], q[6][f]);
+        u[f] = fma(kx + ky + kz, k.KOfac, q[R][f]);
+      }
+      const CksT<T> m = gr_metric_at(k, ai, aj, az);
+      gr_project(k, m, u);
+      const int c = col + az * k.sz;
+#pragma unroll
+      for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
+    }
+#pragma unroll
+    for (int d = 0; d < Q; d++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) q[d][f] = q[d + 1][f];
+  }
+}
+
 }  // namespace coffee
