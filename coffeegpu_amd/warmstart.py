@@ -189,7 +189,8 @@ def train(net: UNet, X, T, n: int, h: int, w: int, epochs: int, batch: int, lr: 
 
 
 def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0, rlc_hi=20.0,
-               levels=4, base=32, epochs=60, batch=8, lr=2e-3, tol=0.01, seed=0, log=None):
+               levels=4, base=32, epochs=60, batch=8, lr=2e-3, tol=0.01, seed=0, log=None,
+               eval_periods=None):
     torch = _torch()
     t_start = time.perf_counter()
     rng = np.random.default_rng(seed)
@@ -223,8 +224,11 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
     t_train = time.perf_counter() - t0
     del X, T
 
-    # 3. evaluation on held-out R_LC: dipole start vs predicted start
-    steps = [steps_of(r) for r in rl_test]
+    # 3. evaluation on held-out R_LC: dipole start vs predicted start, both run
+    # past the training horizon so that the dipole run's late luminosity is a
+    # relaxed reference
+    ep = eval_periods if eval_periods else periods + 1.0
+    steps = [int(math.ceil(ep * 2 * math.pi * r / dt)) for r in rl_test]
     check = max(1, min(steps) // 60)
     dip = Runs(rl_test, nr, nth, steps)
     warm = Runs(rl_test, nr, nth, steps)
@@ -253,18 +257,25 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
                     den0 / max(torch.linalg.vector_norm(relaxed[sl]).item(), 1e-30)))
     res = []
     for b in range(n_test):
-        L_ref = h_dip[b][-1][1]
+        # relaxed reference: the dipole run's mean luminosity over its last quarter
+        tail = [L for _, L in h_dip[b][-max(1, len(h_dip[b]) // 4):]]
+        L_ref = float(np.mean(tail))
         sd = settle_step(h_dip[b], L_ref, tol)
         sw = settle_step(h_warm[b], L_ref, tol)
+        stride = max(1, len(h_dip[b]) // 24)
         res.append({"rlc": float(rl_test[b]), "L_ref": L_ref, "steps_dipole": sd, "steps_warm": sw,
                     "total_steps": steps[b], "field_err_warm": err[b][0],
-                    "field_err_dipole": err[b][1]})
+                    "field_err_dipole": err[b][1],
+                    "L_dipole": [round(L / L_ref, 4) for _, L in h_dip[b][::stride]],
+                    "L_warm": [round(L / L_ref, 4) for _, L in h_warm[b][::stride]],
+                    "L_steps": [st for st, _ in h_dip[b][::stride]]})
     dip.close()
     warm.close()
     gains = [r["steps_dipole"] / r["steps_warm"] for r in res
              if r["steps_dipole"] and r["steps_warm"]]
     return {
         "grid": [nr, nth], "n_train": n_train, "n_test": n_test, "periods": periods,
+        "eval_periods": ep, "tol": tol,
         "rlc_range": [rlc_lo, rlc_hi], "levels": levels, "base": base,
         "params": net.num_params(), "epochs": epochs, "loss_first": losses[0],
         "loss_last": losses[-1], "relax_s": t_relax, "relax_cell_updates_per_s": cells / t_relax,

@@ -21,8 +21,8 @@ def test_settle_step():
 
 
 def test_small_warm_start_experiment():
-    out = experiment(nr=64, nth=32, n_train=32, n_test=4, periods=2.0, rlc_lo=3.0, rlc_hi=6.0,
-                     levels=3, base=16, epochs=40, batch=8, lr=3e-3, seed=1)
+    out = experiment(nr=64, nth=32, n_train=32, n_test=4, periods=3.0, rlc_lo=3.0, rlc_hi=6.0,
+                     levels=3, base=16, epochs=40, batch=8, lr=3e-3, seed=1, eval_periods=5.0)
     print({k: v for k, v in out.items() if k != "runs"})
     for r in out["runs"]:
         print(r)
