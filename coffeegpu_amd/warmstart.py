@@ -173,13 +173,17 @@ def train(net: UNet, X, T, n: int, h: int, w: int, epochs: int, batch: int, lr: 
     losses = []
     for ep in range(epochs):
         tot = 0.0
+        it_in_epoch = 0
         for b in rng.permutation(nb):
             x = X[b * batch * h * w * cin:(b + 1) * batch * h * w * cin]
             t = T[b * batch * h * w * net.cout:(b + 1) * batch * h * w * net.cout]
             out = net.forward(x, batch, h, w)
             loss, d = net.loss_and_grad(out, t, batch, h, w)
             net.backward(d)
-            net.adam_step(lr=lr * (0.5 * (1 + math.cos(math.pi * ep / epochs))))
+            # linear warm-up over the first two epochs, then cosine decay
+            warm = min(1.0, (ep * nb + it_in_epoch + 1) / (2.0 * nb))
+            it_in_epoch += 1
+            net.adam_step(lr=lr * warm * (0.5 * (1 + math.cos(math.pi * ep / epochs))))
             tot += loss
         losses.append(tot / nb)
         if log and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
@@ -189,7 +193,7 @@ def train(net: UNet, X, T, n: int, h: int, w: int, epochs: int, batch: int, lr: 
 
 
 def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0, rlc_hi=20.0,
-               levels=4, base=32, epochs=60, batch=8, lr=2e-3, tol=0.01, seed=0, log=None,
+               levels=4, base=32, epochs=60, batch=8, lr=1e-3, tol=0.01, seed=0, log=None,
                eval_periods=None):
     torch = _torch()
     t_start = time.perf_counter()
@@ -216,6 +220,10 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
     cells = sum(nr * nth * s for s in runs.nsteps)
     runs.close()
     del X0
+    # per-channel scale: every channel of the correction enters the loss at unit RMS
+    Tv = T.view(-1, cout)
+    chan_rms = torch.sqrt((Tv * Tv).mean(dim=0)).clamp_min(1e-12)
+    Tv /= chan_rms
 
     # 2. training
     net = UNet(levels, base, 7, 6, seed=seed)
@@ -240,7 +248,7 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
     for b in range(0, n_test, batch):
         m = min(batch, n_test - b)
         out = net.forward(Xt[b * H * W * cin:(b + m) * H * W * cin], m, H, W)
-        o = out.view(m * H * W, pad16(cout))[:, :cout].reshape(-1)
+        o = (out.view(m * H * W, pad16(cout))[:, :cout] * chan_rms).reshape(-1)
         pred[b * H * W * cout:(b + m) * H * W * cout] = o + base0[b * H * W * cout:(b + m) * H * W * cout]
     unpack_runs(warm, pred, cout)
     h_dip = luminosity_history(dip, check, steps)
@@ -262,8 +270,11 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
         L_ref = float(np.mean(tail))
         sd = settle_step(h_dip[b], L_ref, tol)
         sw = settle_step(h_warm[b], L_ref, tol)
+        by_tol = {str(tt): [settle_step(h_dip[b], L_ref, tt), settle_step(h_warm[b], L_ref, tt)]
+                  for tt in (0.01, 0.02, 0.05, 0.1)}
         stride = max(1, len(h_dip[b]) // 24)
         res.append({"rlc": float(rl_test[b]), "L_ref": L_ref, "steps_dipole": sd, "steps_warm": sw,
+                    "settle_by_tol": by_tol,
                     "total_steps": steps[b], "field_err_warm": err[b][0],
                     "field_err_dipole": err[b][1],
                     "L_dipole": [round(L / L_ref, 4) for _, L in h_dip[b][::stride]],
@@ -273,6 +284,13 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
     warm.close()
     gains = [r["steps_dipole"] / r["steps_warm"] for r in res
              if r["steps_dipole"] and r["steps_warm"]]
+    gain_by_tol = {}
+    for tt in ("0.01", "0.02", "0.05", "0.1"):
+        g = [r["settle_by_tol"][tt][0] / r["settle_by_tol"][tt][1] for r in res
+             if r["settle_by_tol"][tt][0] and r["settle_by_tol"][tt][1]]
+        gain_by_tol[tt] = {"median_gain": float(np.median(g)) if g else None, "n": len(g),
+                           "dipole_settled": sum(1 for r in res if r["settle_by_tol"][tt][0]),
+                           "warm_settled": sum(1 for r in res if r["settle_by_tol"][tt][1])}
     return {
         "grid": [nr, nth], "n_train": n_train, "n_test": n_test, "periods": periods,
         "eval_periods": ep, "tol": tol,
@@ -280,6 +298,7 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
         "params": net.num_params(), "epochs": epochs, "loss_first": losses[0],
         "loss_last": losses[-1], "relax_s": t_relax, "relax_cell_updates_per_s": cells / t_relax,
         "train_s": t_train, "median_gain": float(np.median(gains)) if gains else None,
+        "gain_by_tol": gain_by_tol,
         "n_settled_both": len(gains),
         "median_field_err_warm": float(np.median([r["field_err_warm"] for r in res])),
         "median_field_err_dipole": float(np.median([r["field_err_dipole"] for r in res])),
@@ -299,7 +318,8 @@ def main(argv=None):
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval-periods", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
 
@@ -307,7 +327,8 @@ def main(argv=None):
         print(f"[warmstart] {msg}", file=sys.stderr, flush=True)
 
     out = experiment(a.nr, a.nth, a.train, a.test, a.periods, a.rlc[0], a.rlc[1], a.levels, a.base,
-                     a.epochs, a.batch, a.lr, seed=a.seed, log=log)
+                     a.epochs, a.batch, a.lr, seed=a.seed, log=log,
+                     eval_periods=a.eval_periods or None)
     print(json.dumps(out))
 
 
