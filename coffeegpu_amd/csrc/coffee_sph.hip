@@ -26,11 +26,14 @@ const double kBs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994
                        0.1530572479681};
 const double kCs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345, 0.9582821306784};
 
-// Tile shapes the stage/KO kernels are built for (TX x TY threads).
+// Tile shapes the stage/KO kernels are built for (TX x TY threads); lds = the
+// tile and its halo are staged in LDS (otherwise stencil taps read global
+// memory through the caches).  Every shape gives bitwise the same results.
 struct Tile {
-  int tx, ty;
+  int tx, ty, lds;
 };
-const Tile kTiles[] = {{64, 4}, {64, 8}, {128, 2}, {32, 8}, {256, 1}};
+const Tile kTiles[] = {{64, 4, 0},  {64, 8, 0},  {128, 2, 0}, {32, 8, 0}, {256, 1, 0},
+                       {64, 8, 1},  {64, 16, 1}, {128, 4, 1}, {32, 16, 1}, {64, 4, 1}};
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 struct SphStateBase : SolverState {
@@ -80,7 +83,7 @@ struct SphState : SphStateBase {
     return COFFEE_OK;
   }
 
-  template <int TX, int TY>
+  template <int TX, int TY, bool LDS>
   int launch_step(coffee_ctx* c, hipStream_t s) {
     const int nr = (int)h_runs.size();
     const dim3 gb((max_n0 + 255) / 256, nr);
@@ -96,9 +99,16 @@ struct SphState : SphStateBase {
       const T as = (T)kAs[st], bs = (T)kBs[st], cs = (T)kCs[st];
       const char* name = st == 0 ? "sph_stage_first" : (st == 4 ? "sph_stage_last" : "sph_stage");
       rc = timed(c, name, s, [&] {
-#define SPH_LAUNCH(F, L, D)                                                             \
-  sph_stage_kernel<T, TX, TY, F, L, D><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, \
-                                                                      nitems, src, as, bs, cs)
+#define SPH_LAUNCH(F, L, D)                                                                 \
+  do {                                                                                      \
+    if (LDS)                                                                                \
+      sph_stage_lds_kernel<T, TX, TY, F, L, D><<<grid_blocks, TX * TY, 0, s>>>(             \
+          d_runs, d_items, nitems, src, as, bs, cs);                                        \
+    else                                                                                    \
+      sph_stage_kernel<T, TX, TY, F, L, D><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, \
+                                                                          nitems, src, as,  \
+                                                                          bs, cs);          \
+  } while (0)
         if (st == 0) {
           if (dg) SPH_LAUNCH(true, false, true); else SPH_LAUNCH(true, false, false);
         } else if (st == 4) {
@@ -114,7 +124,8 @@ struct SphState : SphStateBase {
       if (rc) return rc;
     }
     rc = timed(c, "sph_ko", s, [&] {
-      sph_ko_kernel<T, TX, TY><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, nitems, 1);
+      if (LDS) sph_ko_lds_kernel<T, TX, TY><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, nitems, 1);
+      else sph_ko_kernel<T, TX, TY><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, nitems, 1);
     });
     if (rc) return rc;
     rc = timed(c, "sph_axis", s, [&] { sph_axis_kernel<T><<<gb, 256, 0, s>>>(d_runs, nr, 0); });
@@ -126,11 +137,16 @@ struct SphState : SphStateBase {
 
   int step_direct(coffee_ctx* c, hipStream_t s) {
     switch (tile) {
-      case 0: return launch_step<64, 4>(c, s);
-      case 1: return launch_step<64, 8>(c, s);
-      case 2: return launch_step<128, 2>(c, s);
-      case 3: return launch_step<32, 8>(c, s);
-      default: return launch_step<256, 1>(c, s);
+      case 0: return launch_step<64, 4, false>(c, s);
+      case 1: return launch_step<64, 8, false>(c, s);
+      case 2: return launch_step<128, 2, false>(c, s);
+      case 3: return launch_step<32, 8, false>(c, s);
+      case 4: return launch_step<256, 1, false>(c, s);
+      case 5: return launch_step<64, 8, true>(c, s);
+      case 6: return launch_step<64, 16, true>(c, s);
+      case 7: return launch_step<128, 4, true>(c, s);
+      case 8: return launch_step<32, 16, true>(c, s);
+      default: return launch_step<64, 4, true>(c, s);
     }
   }
 
@@ -460,12 +476,21 @@ int autotune_impl(coffee_ctx* c, int trials, hipStream_t s, int* best, double* m
 #define SPH_FIRST(TX, TY)                                                                 \
   sph_stage_kernel<T, TX, TY, true, false, false><<<S->grid_blocks, TX * TY, 0, s>>>(      \
       S->d_runs, S->d_items, S->nitems, 0, T(0), (T)kBs[0], T(0))
+#define SPH_FIRST_LDS(TX, TY)                                                             \
+  sph_stage_lds_kernel<T, TX, TY, true, false, false><<<S->grid_blocks, TX * TY, 0, s>>>(  \
+      S->d_runs, S->d_items, S->nitems, 0, T(0), (T)kBs[0], T(0))
         case 0: SPH_FIRST(64, 4); break;
         case 1: SPH_FIRST(64, 8); break;
         case 2: SPH_FIRST(128, 2); break;
         case 3: SPH_FIRST(32, 8); break;
-        default: SPH_FIRST(256, 1); break;
+        case 4: SPH_FIRST(256, 1); break;
+        case 5: SPH_FIRST_LDS(64, 8); break;
+        case 6: SPH_FIRST_LDS(64, 16); break;
+        case 7: SPH_FIRST_LDS(128, 4); break;
+        case 8: SPH_FIRST_LDS(32, 16); break;
+        default: SPH_FIRST_LDS(64, 4); break;
 #undef SPH_FIRST
+#undef SPH_FIRST_LDS
       }
     };
     launch();

@@ -140,7 +140,160 @@ __device__ __forceinline__ bool sph_cell(const SphItem& it, int dims0g, int& i, 
   return true;
 }
 
-// One LSRK stage for every active run.  src = buffer index read (0/1).
+// One LSRK stage at interior cell (i, j) of run R.  F[f] points at the cell's
+// value of field f in the buffer read (global memory or an LDS tile) and sy
+// is that storage's row stride, so both kernels below run the same
+// arithmetic and give bitwise the same results.
+template <typename T, bool FIRST, bool LAST, bool DIAG>
+__device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j, int src, T As,
+                                                T Bs, T cs, const T* const* F, int sy) {
+  const int c = i + j * R.dims0;
+  const T* Ex = F[0];
+  const T* Ey = F[1];
+  const T* Ez = F[2];
+  const T* Bx = F[3];
+  const T* By = F[4];
+  const T* Bz = F[5];
+  const T* P = F[6];
+  // metric at the cell and its neighbours
+  T rr[5], ss[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    rr[q] = R.r[i - 2 + q];
+    ss[q] = R.sth[j - 2 + q];
+  }
+  const T r = rr[2], sth = ss[2];
+  const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
+  T gm = sph_sqrtg(r, sth);
+  if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
+  // lowered components at x-neighbours (row j) and y-neighbours (column i)
+  auto dxl = [&](const T* f, int comp) {
+    // d/dx of gamma_cc F (gamma depends on r only for c = 1, 2; on r and theta for c = 3)
+    T v[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const T rq = rr[q];
+      const T gq = comp < 2 ? rq * rq : (rq * sth) * (rq * sth);
+      v[q] = q == 2 ? T(0) : gq * f[q - 2];
+    }
+    return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idx12;
+  };
+  auto dyl = [&](const T* f, int comp) {
+    T v[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      const T gq = comp < 2 ? r * r : (r * ss[q]) * (r * ss[q]);
+      v[q] = q == 2 ? T(0) : gq * f[(q - 2) * sy];
+    }
+    return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idy12;
+  };
+  const T rotBx = dyl(Bz, 2) / gm;
+  const T rotEx = dyl(Ez, 2) / gm;
+  const T rotBy = -dxl(Bz, 2) / gm;
+  const T rotBz = (dxl(By, 1) - dyl(Bx, 0)) / gm;
+  const T rotEy = -dxl(Ez, 2) / gm;
+  const T rotEz = (dxl(Ey, 1) - dyl(Ex, 0)) / gm;
+  // div4_sph (.cu:99-124): d/dx(sqrt(g) F^x) + d/dy(sqrt(g) F^y), over sqrt(g)
+  auto divf = [&](const T* fx, const T* fy) {
+    T vx[5], vy[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      vx[q] = q == 2 ? T(0) : sph_sqrtg(rr[q], sth) * fx[q - 2];
+      vy[q] = q == 2 ? T(0) : sph_sqrtg(r, ss[q]) * fy[(q - 2) * sy];
+    }
+    const T tx = fma(T(8), vx[3] - vx[1], vx[0] - vx[4]) * R.idx12;
+    const T ty = fma(T(8), vy[3] - vy[1], vy[0] - vy[4]) * R.idy12;
+    return (tx + ty) / gm;
+  };
+  const T divE = divf(Ex, Ey);
+  const T divB = divf(Bx, By);
+  T u[7] = {Ex[0], Ey[0], Ez[0], Bx[0], By[0], Bz[0], P[0]};
+  const T elx = g1 * u[0], ely = g2 * u[1], elz = g3 * u[2];
+  const T blx = g1 * u[3], bly = g2 * u[4], blz = g3 * u[5];
+  T B2 = u[3] * blx + u[4] * bly + u[5] * blz;
+  if (B2 < TinyOf<T>::v) B2 = TinyOf<T>::v;
+  T Jx = 0, Jy = 0, Jz = 0;
+  if (R.calc_current) {
+    T Jp = (blx * rotBx + bly * rotBy + blz * rotBz) - (elx * rotEx + ely * rotEy + elz * rotEz);
+    if (R.use_edotb) {
+      const T E2 = u[0] * elx + u[1] * ely + u[2] * elz;
+      const T chi2 = B2 - E2;
+      const T EdotB = u[0] * blx + u[1] * bly + u[2] * blz;
+      const T E02 = T(0.5) * (sqrt(chi2 * chi2 + T(4) * EdotB * EdotB) - chi2);
+      Jp += R.damp_gamma * EdotB / R.dt;
+      const T a = divE / gm / (E02 + B2);
+      Jx = a * (ely * blz - elz * bly) + Jp * u[3] / B2;
+      Jy = a * (elz * blx - elx * blz) + Jp * u[4] / B2;
+      Jz = a * (elx * bly - ely * blx) + Jp * u[5] / B2;
+    } else {
+      const T a = divE / gm;
+      Jx = (a * (ely * blz - elz * bly) + Jp * u[3]) / B2;
+      Jy = (a * (elz * blx - elx * blz) + Jp * u[4]) / B2;
+      Jz = (a * (elx * bly - ely * blx) + Jp * u[5]) / B2;
+    }
+  }
+  T Px = 0, Py = 0;
+  if (R.divB_clean) {
+    Px = fma(T(8), P[1] - P[-1], P[-2] - P[2]) * R.idx12 / g1;
+    Py = fma(T(8), P[sy] - P[-sy], P[-2 * sy] - P[2 * sy]) * R.idy12 / g2;
+  }
+  if (DIAG) {
+    R.J[0][c] = Jx;
+    R.J[1][c] = Jy;
+    R.J[2][c] = Jz;
+    R.divE[c] = divE;
+    R.divB[c] = divB;
+  }
+  const T dt = R.dt;
+  T du[7];
+  du[0] = dt * (rotBx - Jx);
+  du[1] = dt * (rotBy - Jy);
+  du[2] = dt * (rotBz - Jz);
+  du[3] = -dt * (rotEx + Px);
+  du[4] = -dt * (rotEy + Py);
+  du[5] = -dt * rotEz;
+  du[6] = -dt * R.ch2 * divB - u[6] / R.tau;
+  if (!FIRST) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) du[q] = fma(As, R.dU[q][c], du[q]);
+  }
+  if (!LAST) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) R.dU[q][c] = du[q];
+  }
+  const T x = R.x0 + R.dx * T(i - R.g0);
+  if (FIRST && R.pml_on && x > R.xh) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) R.Un[q][c] = u[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 7; q++) u[q] = fma(Bs, du[q], u[q]);
+  sph_project(R, g1, g2, g3, u);
+  const T th = R.y0 + R.dy * T(j - R.g1);
+  const T tstage = (T)R.t + cs * dt;
+  sph_pulsar(R, r, th, sth, g1, g2, g3, tstage, u);
+  if (LAST && R.pml_on && x > R.xh) {
+    T sig = T(0);
+    if (r > R.rh) {
+      const T q = (r - R.rh) / R.pmllen;
+      sig = R.sigpml * (q * q * q);
+    }
+    if (sig > TinyOf<T>::v) {
+      const T e = exp(-sig), f = (T(1) - e) / sig;
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        const T un = R.Un[q][c];
+        u[q] = e * un + f * (u[q] - un);
+      }
+    }
+  }
+  T* const* out = R.U[src ^ 1];
+#pragma unroll
+  for (int q = 0; q < 7; q++) out[q][c] = u[q];
+}
+
+// One LSRK stage for every active run, stencil operands read from global
+// memory (through the caches).  src = buffer index read (0/1).
 template <typename T, int TX, int TY, bool FIRST, bool LAST, bool DIAG>
 __global__ void __launch_bounds__(TX * TY)
     sph_stage_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items, int nitems,
@@ -153,156 +306,101 @@ __global__ void __launch_bounds__(TX * TY)
     const int ii = it.i0 + li, jj = it.j0 + lj;  // interior-relative
     if (ii >= R.n0 || jj >= R.n1) continue;
     const int i = ii + R.g0, j = jj + R.g1;
-    const int s = R.dims0;
-    const int c = i + j * s;
-    const T* Ex = R.U[src][0];
-    const T* Ey = R.U[src][1];
-    const T* Ez = R.U[src][2];
-    const T* Bx = R.U[src][3];
-    const T* By = R.U[src][4];
-    const T* Bz = R.U[src][5];
-    const T* P = R.U[src][6];
-    // metric at the cell and its neighbours
-    T rr[5], ss[5];
+    const int c = i + j * R.dims0;
+    const T* F[7];
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-      rr[q] = R.r[i - 2 + q];
-      ss[q] = R.sth[j - 2 + q];
+    for (int f = 0; f < 7; f++) F[f] = R.U[src][f] + c;
+    sph_stage_point<T, FIRST, LAST, DIAG>(R, i, j, src, As, Bs, cs, F, R.dims0);
+  }
+}
+
+// Stage a TX x TY tile of all 7 fields plus a RH-cell halo of run R into LDS
+// (rows of TX + 2 RH).  Cells beyond the array (partial tiles at the far
+// edges) are clamped to the last column/row; they feed no result.
+template <typename T, int TX, int TY, int RH>
+__device__ __forceinline__ void sph_fill_tile(const SphRun<T>& R, const SphItem& it, int src,
+                                              T (*lds)[(TX + 2 * RH) * (TY + 2 * RH)]) {
+  constexpr int LXW = TX + 2 * RH, NCELL = LXW * (TY + 2 * RH);
+  for (int idx = threadIdx.x; idx < NCELL; idx += TX * TY) {
+    const int lx = idx % LXW, ly = idx / LXW;
+    int gi = it.i0 + lx - RH + R.g0, gj = it.j0 + ly - RH + R.g1;
+    gi = gi > R.dims0 - 1 ? R.dims0 - 1 : gi;
+    gj = gj > R.dims1 - 1 ? R.dims1 - 1 : gj;
+    const int g = gi + gj * R.dims0;
+#pragma unroll
+    for (int f = 0; f < 7; f++) lds[f][idx] = R.U[src][f][g];
+  }
+}
+
+// The same stage with the tile and its 2-cell halo staged in LDS: one load
+// per field and cell instead of one per stencil tap.
+template <typename T, int TX, int TY, bool FIRST, bool LAST, bool DIAG>
+__global__ void __launch_bounds__(TX * TY)
+    sph_stage_lds_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items,
+                         int nitems, int src, T As, T Bs, T cs) {
+  constexpr int RH = 2, LXW = TX + 2 * RH;
+  __shared__ T lds[7][LXW * (TY + 2 * RH)];
+  const int li = (int)(threadIdx.x % TX), lj = (int)(threadIdx.x / TX);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const SphItem it = items[w];
+    const SphRun<T>& R = runs[it.run];
+    if (!R.active) continue;  // uniform: the whole workgroup works on one item
+    sph_fill_tile<T, TX, TY, RH>(R, it, src, lds);
+    __syncthreads();
+    const int ii = it.i0 + li, jj = it.j0 + lj;
+    if (ii < R.n0 && jj < R.n1) {
+      const int slot = (lj + RH) * LXW + li + RH;
+      const T* F[7];
+#pragma unroll
+      for (int f = 0; f < 7; f++) F[f] = &lds[f][slot];
+      sph_stage_point<T, FIRST, LAST, DIAG>(R, ii + R.g0, jj + R.g1, src, As, Bs, cs, F, LXW);
     }
-    const T r = rr[2], sth = ss[2];
-    const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
-    T gm = sph_sqrtg(r, sth);
-    if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
-    // lowered components at x-neighbours (row j) and y-neighbours (column i)
-    auto dxl = [&](const T* f, int comp) {
-      // d/dx of gamma_cc F (gamma depends on r only for c = 1, 2; on r and theta for c = 3)
-      T v[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) {
-        const T rq = rr[q];
-        const T gq = comp < 2 ? rq * rq : (rq * sth) * (rq * sth);
-        v[q] = q == 2 ? T(0) : gq * f[c + q - 2];
-      }
-      return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idx12;
-    };
-    auto dyl = [&](const T* f, int comp) {
-      T v[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) {
-        const T gq = comp < 2 ? r * r : (r * ss[q]) * (r * ss[q]);
-        v[q] = q == 2 ? T(0) : gq * f[c + (q - 2) * s];
-      }
-      return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idy12;
-    };
-    const T rotBx = dyl(Bz, 2) / gm;
-    const T rotEx = dyl(Ez, 2) / gm;
-    const T rotBy = -dxl(Bz, 2) / gm;
-    const T rotBz = (dxl(By, 1) - dyl(Bx, 0)) / gm;
-    const T rotEy = -dxl(Ez, 2) / gm;
-    const T rotEz = (dxl(Ey, 1) - dyl(Ex, 0)) / gm;
-    // div4_sph (.cu:99-124): d/dx(sqrt(g) F^x) + d/dy(sqrt(g) F^y), over sqrt(g)
-    auto divf = [&](const T* fx, const T* fy) {
-      T vx[5], vy[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) {
-        vx[q] = q == 2 ? T(0) : sph_sqrtg(rr[q], sth) * fx[c + q - 2];
-        vy[q] = q == 2 ? T(0) : sph_sqrtg(r, ss[q]) * fy[c + (q - 2) * s];
-      }
-      const T tx = fma(T(8), vx[3] - vx[1], vx[0] - vx[4]) * R.idx12;
-      const T ty = fma(T(8), vy[3] - vy[1], vy[0] - vy[4]) * R.idy12;
-      return (tx + ty) / gm;
-    };
-    const T divE = divf(Ex, Ey);
-    const T divB = divf(Bx, By);
-    T u[7] = {Ex[c], Ey[c], Ez[c], Bx[c], By[c], Bz[c], P[c]};
-    const T elx = g1 * u[0], ely = g2 * u[1], elz = g3 * u[2];
-    const T blx = g1 * u[3], bly = g2 * u[4], blz = g3 * u[5];
-    T B2 = u[3] * blx + u[4] * bly + u[5] * blz;
-    if (B2 < TinyOf<T>::v) B2 = TinyOf<T>::v;
-    T Jx = 0, Jy = 0, Jz = 0;
-    if (R.calc_current) {
-      T Jp = (blx * rotBx + bly * rotBy + blz * rotBz) - (elx * rotEx + ely * rotEy + elz * rotEz);
-      if (R.use_edotb) {
-        const T E2 = u[0] * elx + u[1] * ely + u[2] * elz;
-        const T chi2 = B2 - E2;
-        const T EdotB = u[0] * blx + u[1] * bly + u[2] * blz;
-        const T E02 = T(0.5) * (sqrt(chi2 * chi2 + T(4) * EdotB * EdotB) - chi2);
-        Jp += R.damp_gamma * EdotB / R.dt;
-        const T a = divE / gm / (E02 + B2);
-        Jx = a * (ely * blz - elz * bly) + Jp * u[3] / B2;
-        Jy = a * (elz * blx - elx * blz) + Jp * u[4] / B2;
-        Jz = a * (elx * bly - ely * blx) + Jp * u[5] / B2;
-      } else {
-        const T a = divE / gm;
-        Jx = (a * (ely * blz - elz * bly) + Jp * u[3]) / B2;
-        Jy = (a * (elz * blx - elx * blz) + Jp * u[4]) / B2;
-        Jz = (a * (elx * bly - ely * blx) + Jp * u[5]) / B2;
-      }
-    }
-    T Px = 0, Py = 0;
-    if (R.divB_clean) {
-      Px = fma(T(8), P[c + 1] - P[c - 1], P[c - 2] - P[c + 2]) * R.idx12 / g1;
-      Py = fma(T(8), P[c + s] - P[c - s], P[c - 2 * s] - P[c + 2 * s]) * R.idy12 / g2;
-    }
-    if (DIAG) {
-      R.J[0][c] = Jx;
-      R.J[1][c] = Jy;
-      R.J[2][c] = Jz;
-      R.divE[c] = divE;
-      R.divB[c] = divB;
-    }
-    const T dt = R.dt;
-    T du[7];
-    du[0] = dt * (rotBx - Jx);
-    du[1] = dt * (rotBy - Jy);
-    du[2] = dt * (rotBz - Jz);
-    du[3] = -dt * (rotEx + Px);
-    du[4] = -dt * (rotEy + Py);
-    du[5] = -dt * rotEz;
-    du[6] = -dt * R.ch2 * divB - u[6] / R.tau;
-    if (!FIRST) {
-#pragma unroll
-      for (int q = 0; q < 7; q++) du[q] = fma(As, R.dU[q][c], du[q]);
-    }
-    if (!LAST) {
-#pragma unroll
-      for (int q = 0; q < 7; q++) R.dU[q][c] = du[q];
-    }
-    const T x = R.x0 + R.dx * T(i - R.g0);
-    if (FIRST && R.pml_on && x > R.xh) {
-#pragma unroll
-      for (int q = 0; q < 7; q++) R.Un[q][c] = u[q];
-    }
-#pragma unroll
-    for (int q = 0; q < 7; q++) u[q] = fma(Bs, du[q], u[q]);
-    sph_project(R, g1, g2, g3, u);
-    const T th = R.y0 + R.dy * T(j - R.g1);
-    const T tstage = (T)R.t + cs * dt;
-    sph_pulsar(R, r, th, sth, g1, g2, g3, tstage, u);
-    if (LAST && R.pml_on && x > R.xh) {
-      T sig = T(0);
-      if (r > R.rh) {
-        const T q = (r - R.rh) / R.pmllen;
-        sig = R.sigpml * (q * q * q);
-      }
-      if (sig > TinyOf<T>::v) {
-        const T e = exp(-sig), f = (T(1) - e) / sig;
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-          const T un = R.Un[q][c];
-          u[q] = e * un + f * (u[q] - un);
-        }
-      }
-    }
-    T* const* out = R.U[src ^ 1];
-#pragma unroll
-    for (int q = 0; q < 7; q++) out[q][c] = u[q];
+    __syncthreads();
   }
 }
 
 // Kreiss-Oliger on all 7 fields (kernel_KO_step1/2_sph, .cu:444-555):
 // KO_2d6 (sqrt(g)-weighted, KO_geometry) or KO_2d0, then projections and the
-// pulsar boundary at t + dt.
+// pulsar boundary at t + dt.  F, sy as in sph_stage_point.
+template <typename T>
+__device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, int src,
+                                             const T* const* F, int sy) {
+  const int c = i + j * R.dims0;
+  T rr[7], ss[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    rr[q] = R.r[i - 3 + q];
+    ss[q] = R.sth[j - 3 + q];
+  }
+  const T r = rr[3], sth = ss[3];
+  T gm = sph_sqrtg(r, sth);
+  if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
+  T wx[7], wy[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    wx[q] = R.KO_geometry ? sph_sqrtg(rr[q], sth) : T(1);
+    wy[q] = R.KO_geometry ? sph_sqrtg(r, ss[q]) : T(1);
+  }
+  const T den = R.KO_geometry ? gm : T(1);
+  T u[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) {
+    const T* p = F[f];
+    const T kx = d6v(wx[0] * p[-3], wx[1] * p[-2], wx[2] * p[-1], wx[3] * p[0], wx[4] * p[1],
+                     wx[5] * p[2], wx[6] * p[3]);
+    const T ky = d6v(wy[0] * p[-3 * sy], wy[1] * p[-2 * sy], wy[2] * p[-sy], wy[3] * p[0],
+                     wy[4] * p[sy], wy[5] * p[2 * sy], wy[6] * p[3 * sy]);
+    u[f] = fma((kx + ky) / den, R.KOfac, p[0]);
+  }
+  const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
+  sph_project(R, g1, g2, g3, u);
+  const T th = R.y0 + R.dy * T(j - R.g1);
+  sph_pulsar(R, r, th, sth, g1, g2, g3, (T)R.t + R.dt, u);
+  T* const* out = R.U[src ^ 1];
+#pragma unroll
+  for (int q = 0; q < 7; q++) out[q][c] = u[q];
+}
+
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY)
     sph_ko_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items, int nitems,
@@ -314,41 +412,36 @@ __global__ void __launch_bounds__(TX * TY)
     const int ii = it.i0 + (int)(threadIdx.x % TX), jj = it.j0 + (int)(threadIdx.x / TX);
     if (ii >= R.n0 || jj >= R.n1) continue;
     const int i = ii + R.g0, j = jj + R.g1;
-    const int s = R.dims0;
-    const int c = i + j * s;
-    T rr[7], ss[7];
+    const int c = i + j * R.dims0;
+    const T* F[7];
 #pragma unroll
-    for (int q = 0; q < 7; q++) {
-      rr[q] = R.r[i - 3 + q];
-      ss[q] = R.sth[j - 3 + q];
+    for (int f = 0; f < 7; f++) F[f] = R.U[src][f] + c;
+    sph_ko_point<T>(R, i, j, src, F, R.dims0);
+  }
+}
+
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY)
+    sph_ko_lds_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items, int nitems,
+                      int src) {
+  constexpr int RH = 3, LXW = TX + 2 * RH;
+  __shared__ T lds[7][LXW * (TY + 2 * RH)];
+  const int li = (int)(threadIdx.x % TX), lj = (int)(threadIdx.x / TX);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const SphItem it = items[w];
+    const SphRun<T>& R = runs[it.run];
+    if (!R.active) continue;  // uniform: the whole workgroup works on one item
+    sph_fill_tile<T, TX, TY, RH>(R, it, src, lds);
+    __syncthreads();
+    const int ii = it.i0 + li, jj = it.j0 + lj;
+    if (ii < R.n0 && jj < R.n1) {
+      const int slot = (lj + RH) * LXW + li + RH;
+      const T* F[7];
+#pragma unroll
+      for (int f = 0; f < 7; f++) F[f] = &lds[f][slot];
+      sph_ko_point<T>(R, ii + R.g0, jj + R.g1, src, F, LXW);
     }
-    const T r = rr[3], sth = ss[3];
-    T gm = sph_sqrtg(r, sth);
-    if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
-    T wx[7], wy[7];
-#pragma unroll
-    for (int q = 0; q < 7; q++) {
-      wx[q] = R.KO_geometry ? sph_sqrtg(rr[q], sth) : T(1);
-      wy[q] = R.KO_geometry ? sph_sqrtg(r, ss[q]) : T(1);
-    }
-    const T den = R.KO_geometry ? gm : T(1);
-    T u[7];
-#pragma unroll
-    for (int f = 0; f < 7; f++) {
-      const T* p = R.U[src][f];
-      const T kx = d6v(wx[0] * p[c - 3], wx[1] * p[c - 2], wx[2] * p[c - 1], wx[3] * p[c],
-                       wx[4] * p[c + 1], wx[5] * p[c + 2], wx[6] * p[c + 3]);
-      const T ky = d6v(wy[0] * p[c - 3 * s], wy[1] * p[c - 2 * s], wy[2] * p[c - s], wy[3] * p[c],
-                       wy[4] * p[c + s], wy[5] * p[c + 2 * s], wy[6] * p[c + 3 * s]);
-      u[f] = fma((kx + ky) / den, R.KOfac, p[c]);
-    }
-    const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
-    sph_project(R, g1, g2, g3, u);
-    const T th = R.y0 + R.dy * T(j - R.g1);
-    sph_pulsar(R, r, th, sth, g1, g2, g3, (T)R.t + R.dt, u);
-    T* const* out = R.U[src ^ 1];
-#pragma unroll
-    for (int q = 0; q < 7; q++) out[q][c] = u[q];
+    __syncthreads();
   }
 }
 
