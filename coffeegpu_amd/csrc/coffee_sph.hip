@@ -259,16 +259,28 @@ int make_run(coffee_ctx* c, SphState<T>* S, const coffee_grid& g, const coffee_p
   R.KO_geometry = p.KO_geometry;
   R.pml_on = p.pml[0] > 0;
   // metric tables r(i) = exp(x_i), sin(theta_j) at the node positions
-  std::vector<T> r(g.dims[0]), sth(g.dims[1]);
-  for (int i = 0; i < g.dims[0]; i++) r[i] = (T)std::exp((double)(R.x0 + R.dx * T(i - R.g0)));
+  std::vector<T> r(g.dims[0]), sth(g.dims[1]), r3(g.dims[0]), ir2(g.dims[0]);
+  for (int i = 0; i < g.dims[0]; i++) {
+    const double rd = std::exp((double)(R.x0 + R.dx * T(i - R.g0)));
+    r[i] = (T)rd;
+    r3[i] = (T)(rd * rd * rd);
+    ir2[i] = (T)(1.0 / (rd * rd));
+  }
   for (int j = 0; j < g.dims[1]; j++) sth[j] = (T)std::sin((double)(R.y0 + R.dy * T(j - R.g1)));
-  T *dr = nullptr, *ds = nullptr;
+  T *dr = nullptr, *ds = nullptr, *dr3 = nullptr, *dir2 = nullptr;
   if ((rc = alloc_owned(c, S, r.size(), &dr))) return rc;
   if ((rc = alloc_owned(c, S, sth.size(), &ds))) return rc;
+  if ((rc = alloc_owned(c, S, r3.size(), &dr3))) return rc;
+  if ((rc = alloc_owned(c, S, ir2.size(), &dir2))) return rc;
   HIP_TRY(c, hipMemcpy(dr, r.data(), r.size() * sizeof(T), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(ds, sth.data(), sth.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(dr3, r3.data(), r3.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(dir2, ir2.data(), ir2.size() * sizeof(T), hipMemcpyHostToDevice));
   R.r = dr;
   R.sth = ds;
+  R.r3 = dr3;
+  R.ir2 = dir2;
+  R.inv_tau = (T)(1.0 / p.tau);
   const size_t n = (size_t)g.dims[0] * g.dims[1];
   for (int q = 0; q < 7; q++)
     if ((rc = alloc_owned(c, S, n, &R.U[1][q]))) return rc;

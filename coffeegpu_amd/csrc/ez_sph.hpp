@@ -39,6 +39,9 @@ struct SphRun {
   int calc_current, use_edotb, divB_clean, clean_ep, check_egb, KO_geometry, pml_on;
   const T* r;     // r(i) = exp(x_i), dims0 entries
   const T* sth;   // sin(theta_j), dims1 entries
+  const T* r3;    // r(i)^3: sqrt(gamma) = r^3 sin(theta)
+  const T* ir2;   // 1 / r(i)^2 = gamma^11 = gamma^22
+  T inv_tau;
   T* U[2][7];     // [0] = caller's state, [1] = ping-pong partner
   T* dU[7];
   T* Un[7];       // step-start values in the absorbing layer
@@ -164,8 +167,15 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
   }
   const T r = rr[2], sth = ss[2];
   const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
-  T gm = sph_sqrtg(r, sth);
+  // sqrt(gamma) = r^3 |sin(theta)| from the per-run tables (metric_sph.h:124-133
+  // takes the square root of g11 g22 g33, which is positive also in the guard
+  // rows beyond the axis, where sin(theta) < 0)
+  T r3[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) r3[q] = R.r3[i - 2 + q];
+  T gm = r3[2] * fabs(sth);
   if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
+  const T igm = T(1) / gm;
   // lowered components at x-neighbours (row j) and y-neighbours (column i)
   auto dxl = [&](const T* f, int comp) {
     // d/dx of gamma_cc F (gamma depends on r only for c = 1, 2; on r and theta for c = 3)
@@ -187,23 +197,23 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
     }
     return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idy12;
   };
-  const T rotBx = dyl(Bz, 2) / gm;
-  const T rotEx = dyl(Ez, 2) / gm;
-  const T rotBy = -dxl(Bz, 2) / gm;
-  const T rotBz = (dxl(By, 1) - dyl(Bx, 0)) / gm;
-  const T rotEy = -dxl(Ez, 2) / gm;
-  const T rotEz = (dxl(Ey, 1) - dyl(Ex, 0)) / gm;
+  const T rotBx = dyl(Bz, 2) * igm;
+  const T rotEx = dyl(Ez, 2) * igm;
+  const T rotBy = -dxl(Bz, 2) * igm;
+  const T rotBz = (dxl(By, 1) - dyl(Bx, 0)) * igm;
+  const T rotEy = -dxl(Ez, 2) * igm;
+  const T rotEz = (dxl(Ey, 1) - dyl(Ex, 0)) * igm;
   // div4_sph (.cu:99-124): d/dx(sqrt(g) F^x) + d/dy(sqrt(g) F^y), over sqrt(g)
   auto divf = [&](const T* fx, const T* fy) {
     T vx[5], vy[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      vx[q] = q == 2 ? T(0) : sph_sqrtg(rr[q], sth) * fx[q - 2];
-      vy[q] = q == 2 ? T(0) : sph_sqrtg(r, ss[q]) * fy[(q - 2) * sy];
+      vx[q] = q == 2 ? T(0) : (r3[q] * fabs(sth)) * fx[q - 2];
+      vy[q] = q == 2 ? T(0) : (r3[2] * fabs(ss[q])) * fy[(q - 2) * sy];
     }
     const T tx = fma(T(8), vx[3] - vx[1], vx[0] - vx[4]) * R.idx12;
     const T ty = fma(T(8), vy[3] - vy[1], vy[0] - vy[4]) * R.idy12;
-    return (tx + ty) / gm;
+    return (tx + ty) * igm;
   };
   const T divE = divf(Ex, Ey);
   const T divB = divf(Bx, By);
@@ -212,6 +222,7 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
   const T blx = g1 * u[3], bly = g2 * u[4], blz = g3 * u[5];
   T B2 = u[3] * blx + u[4] * bly + u[5] * blz;
   if (B2 < TinyOf<T>::v) B2 = TinyOf<T>::v;
+  const T iB2 = T(1) / B2;
   T Jx = 0, Jy = 0, Jz = 0;
   if (R.calc_current) {
     T Jp = (blx * rotBx + bly * rotBy + blz * rotBz) - (elx * rotEx + ely * rotEy + elz * rotEz);
@@ -221,21 +232,23 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
       const T EdotB = u[0] * blx + u[1] * bly + u[2] * blz;
       const T E02 = T(0.5) * (sqrt(chi2 * chi2 + T(4) * EdotB * EdotB) - chi2);
       Jp += R.damp_gamma * EdotB / R.dt;
-      const T a = divE / gm / (E02 + B2);
-      Jx = a * (ely * blz - elz * bly) + Jp * u[3] / B2;
-      Jy = a * (elz * blx - elx * blz) + Jp * u[4] / B2;
-      Jz = a * (elx * bly - ely * blx) + Jp * u[5] / B2;
+      const T a = divE * igm / (E02 + B2);
+      const T JpB = Jp * iB2;
+      Jx = a * (ely * blz - elz * bly) + JpB * u[3];
+      Jy = a * (elz * blx - elx * blz) + JpB * u[4];
+      Jz = a * (elx * bly - ely * blx) + JpB * u[5];
     } else {
-      const T a = divE / gm;
-      Jx = (a * (ely * blz - elz * bly) + Jp * u[3]) / B2;
-      Jy = (a * (elz * blx - elx * blz) + Jp * u[4]) / B2;
-      Jz = (a * (elx * bly - ely * blx) + Jp * u[5]) / B2;
+      const T a = divE * igm;
+      Jx = (a * (ely * blz - elz * bly) + Jp * u[3]) * iB2;
+      Jy = (a * (elz * blx - elx * blz) + Jp * u[4]) * iB2;
+      Jz = (a * (elx * bly - ely * blx) + Jp * u[5]) * iB2;
     }
   }
   T Px = 0, Py = 0;
   if (R.divB_clean) {
-    Px = fma(T(8), P[1] - P[-1], P[-2] - P[2]) * R.idx12 / g1;
-    Py = fma(T(8), P[sy] - P[-sy], P[-2 * sy] - P[2 * sy]) * R.idy12 / g2;
+    const T ig = R.ir2[i];
+    Px = fma(T(8), P[1] - P[-1], P[-2] - P[2]) * R.idx12 * ig;
+    Py = fma(T(8), P[sy] - P[-sy], P[-2 * sy] - P[2 * sy]) * R.idy12 * ig;
   }
   if (DIAG) {
     R.J[0][c] = Jx;
@@ -252,7 +265,7 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
   du[3] = -dt * (rotEx + Px);
   du[4] = -dt * (rotEy + Py);
   du[5] = -dt * rotEz;
-  du[6] = -dt * R.ch2 * divB - u[6] / R.tau;
+  du[6] = -dt * R.ch2 * divB - u[6] * R.inv_tau;
   if (!FIRST) {
 #pragma unroll
     for (int q = 0; q < 7; q++) du[q] = fma(As, R.dU[q][c], du[q]);
@@ -373,15 +386,16 @@ __device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, i
     ss[q] = R.sth[j - 3 + q];
   }
   const T r = rr[3], sth = ss[3];
-  T gm = sph_sqrtg(r, sth);
+  const T r3c = R.r3[i];
+  T gm = r3c * fabs(sth);
   if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
   T wx[7], wy[7];
 #pragma unroll
   for (int q = 0; q < 7; q++) {
-    wx[q] = R.KO_geometry ? sph_sqrtg(rr[q], sth) : T(1);
-    wy[q] = R.KO_geometry ? sph_sqrtg(r, ss[q]) : T(1);
+    wx[q] = R.KO_geometry ? R.r3[i - 3 + q] * fabs(sth) : T(1);
+    wy[q] = R.KO_geometry ? r3c * fabs(ss[q]) : T(1);
   }
-  const T den = R.KO_geometry ? gm : T(1);
+  const T iden = R.KO_geometry ? T(1) / gm : T(1);
   T u[7];
 #pragma unroll
   for (int f = 0; f < 7; f++) {
@@ -390,7 +404,7 @@ __device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, i
                      wx[5] * p[2], wx[6] * p[3]);
     const T ky = d6v(wy[0] * p[-3 * sy], wy[1] * p[-2 * sy], wy[2] * p[-sy], wy[3] * p[0],
                      wy[4] * p[sy], wy[5] * p[2 * sy], wy[6] * p[3 * sy]);
-    u[f] = fma((kx + ky) / den, R.KOfac, p[0]);
+    u[f] = fma((kx + ky) * iden, R.KOfac, p[0]);
   }
   const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
   sph_project(R, g1, g2, g3, u);

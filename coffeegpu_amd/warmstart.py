@@ -48,7 +48,13 @@ def _torch():
 
 def run_params(nr: int, nth: int, rlc: float, r_out: float = 60.0, r_abs: float = 48.0) -> SimParams:
     """Star R = 1, b0 = 1, x = ln r from -0.1 to ln r_out, theta in [0, pi],
-    absorbing layer r > r_abs, Omega = 1/R_LC, dt = 0.4 min(dx, dtheta)."""
+    absorbing layer r > r_abs, Omega = 1/R_LC, dt = 0.4 min(dx, dtheta).
+
+    Kreiss-Oliger uses the flat 2D stencil (KO_geometry off): with the
+    sqrt(gamma)-weighted form, whose weights vanish on the axis, the open
+    field of runs with R_LC >~ 8 dissipates and the spin-down luminosity
+    decays for as long as the run lasts, so there is no steady state to
+    learn or to reach."""
     x0 = -0.1
     size_x = math.log(r_out) - x0
     dx, dth = size_x / nr, math.pi / nth
@@ -56,7 +62,7 @@ def run_params(nr: int, nth: int, rlc: float, r_out: float = 60.0, r_abs: float 
     return SimParams(dt=0.4 * min(dx, dth), N=[nr, nth, 1], guard=[3, 4, 0], lower=[x0, 0.0, 0.0],
                      size=[size_x, math.pi, 0.0], nodes=[1, 1, 1],
                      periodic_boundary=[False, False, False], pml=[pml, 0, 0], pmllen=10,
-                     sigpml=1.0, divB_clean=True, ch2=1.0, tau=2.0, KOeps=0.1, KO_geometry=True,
+                     sigpml=1.0, divB_clean=True, ch2=1.0, tau=2.0, KOeps=0.1, KO_geometry=False,
                      use_edotb_damping=False, calc_current=True, clean_ep=True, check_egb=True,
                      problem=0, radius=1.0, omega=1.0 / rlc, b0=1.0, dw0=0.0, dw1=0.0)
 
