@@ -300,7 +300,10 @@ def run_gr(args):
                    "cells": cells, "schedule": "tiled" if args.variant != 1 else "point+stage",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gr_stage",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (load_traffic(f"gr_stage_{n}_{args.dtype}_traffic.json") or {}).get(
+                         "bytes_per_launch") if args.variant != 1 else None,
+                     "kernel": "gr_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernels": _kernel_table(timing, GR_SCALARS, esz, cells),
         "energy_B_after": W,

@@ -77,10 +77,42 @@ struct SphState : SphStateBase {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    // persistent launch: a few workgroups per CU loop over the work list
-    const int per_cu = std::max(1, 2048 / (t.tx * t.ty));
+    // persistent launch: exactly as many workgroups as can be resident at once
+    // (a larger grid would leave the extra workgroups to start only when
+    // resident ones have finished their whole share of the work list)
+    const int per_cu = std::max(1, resident_per_cu());
     grid_blocks = std::max(1, std::min(nitems, cus * per_cu));
     return COFFEE_OK;
+  }
+
+  template <int TX, int TY, bool LDS>
+  static int occupancy() {
+    int nb = 0;
+    hipError_t e;
+    if (LDS)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void*>(sph_stage_lds_kernel<T, TX, TY, false, false, false>),
+          TX * TY, 0);
+    else
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void*>(sph_stage_kernel<T, TX, TY, false, false, false>),
+          TX * TY, 0);
+    return e == hipSuccess ? nb : 2048 / (TX * TY);
+  }
+
+  int resident_per_cu() const {
+    switch (tile) {
+      case 0: return occupancy<64, 4, false>();
+      case 1: return occupancy<64, 8, false>();
+      case 2: return occupancy<128, 2, false>();
+      case 3: return occupancy<32, 8, false>();
+      case 4: return occupancy<256, 1, false>();
+      case 5: return occupancy<64, 8, true>();
+      case 6: return occupancy<64, 16, true>();
+      case 7: return occupancy<128, 4, true>();
+      case 8: return occupancy<32, 16, true>();
+      default: return occupancy<64, 4, true>();
+    }
   }
 
   template <int TX, int TY, bool LDS>
