@@ -26,15 +26,80 @@ const double kBs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994
                        0.1530572479681};
 const double kCs[5] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345, 0.9582821306784};
 
-// Tile shapes the stage/KO kernels are built for (TX x TY threads); lds = the
-// tile and its halo are staged in LDS (otherwise stencil taps read global
-// memory through the caches).  Every shape gives bitwise the same results.
+// Tile shapes the stage/KO kernels are built for.  kind 0: TX x TY threads,
+// one cell each, stencil taps read global memory through the caches; kind 1:
+// the same with the tile and its halo staged in LDS; kind 2: TX threads march
+// a band of TX radial cells over TY theta rows (register queue + LDS row).
+// Every shape gives bitwise the same results.
 struct Tile {
-  int tx, ty, lds;
+  int tx, ty, kind;
 };
-const Tile kTiles[] = {{64, 4, 0},  {64, 8, 0},  {128, 2, 0}, {32, 8, 0}, {256, 1, 0},
-                       {64, 8, 1},  {64, 16, 1}, {128, 4, 1}, {32, 16, 1}, {64, 4, 1}};
+template <int TX, int TY, int KIND>
+struct TileTag {
+  static constexpr int tx = TX, ty = TY, kind = KIND;
+};
+#define COFFEE_SPH_TILES(X)                                                                      \
+  X(0, 64, 4, 0) X(1, 64, 8, 0) X(2, 128, 2, 0) X(3, 32, 8, 0) X(4, 256, 1, 0) X(5, 64, 8, 1)    \
+  X(6, 64, 16, 1) X(7, 128, 4, 1) X(8, 32, 16, 1) X(9, 64, 4, 1) X(10, 128, 64, 2)              \
+  X(11, 256, 32, 2) X(12, 256, 64, 2) X(13, 256, 128, 2) X(14, 512, 64, 2) X(15, 128, 128, 2)
+#define COFFEE_SPH_TILE_ENTRY(n, tx, ty, k) {tx, ty, k},
+const Tile kTiles[] = {COFFEE_SPH_TILES(COFFEE_SPH_TILE_ENTRY)};
+#undef COFFEE_SPH_TILE_ENTRY
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+// Calls f(TileTag<tx, ty, kind>{}) for tile index `tile`.
+template <typename F>
+int with_tile(int tile, F&& f) {
+  switch (tile) {
+#define COFFEE_SPH_TILE_CASE(n, tx, ty, k) \
+  case n:                                  \
+    return f(TileTag<tx, ty, k>{});
+    COFFEE_SPH_TILES(COFFEE_SPH_TILE_CASE)
+#undef COFFEE_SPH_TILE_CASE
+  }
+  return COFFEE_ERR_INVALID;
+}
+
+// The first-stage / middle / last-stage kernels and the KO kernel of a tile.
+template <typename T, typename Tag, bool F, bool L, bool D>
+void launch_stage(int blocks, hipStream_t s, SphRun<T>* runs, const SphItem* items, int n,
+                  int src, T as, T bs, T cs) {
+  constexpr int TX = Tag::tx, TY = Tag::ty;
+  if constexpr (Tag::kind == 2)
+    sph_stage_march_kernel<T, TX, TY, F, L, D><<<blocks, TX, 0, s>>>(runs, items, n, src, as, bs, cs);
+  else if constexpr (Tag::kind == 1)
+    sph_stage_lds_kernel<T, TX, TY, F, L, D><<<blocks, TX * TY, 0, s>>>(runs, items, n, src, as, bs, cs);
+  else
+    sph_stage_kernel<T, TX, TY, F, L, D><<<blocks, TX * TY, 0, s>>>(runs, items, n, src, as, bs, cs);
+}
+
+template <typename T, typename Tag>
+void launch_ko(int blocks, hipStream_t s, SphRun<T>* runs, const SphItem* items, int n, int src) {
+  constexpr int TX = Tag::tx, TY = Tag::ty;
+  if constexpr (Tag::kind == 2)
+    sph_ko_march_kernel<T, TX, TY><<<blocks, TX, 0, s>>>(runs, items, n, src);
+  else if constexpr (Tag::kind == 1)
+    sph_ko_lds_kernel<T, TX, TY><<<blocks, TX * TY, 0, s>>>(runs, items, n, src);
+  else
+    sph_ko_kernel<T, TX, TY><<<blocks, TX * TY, 0, s>>>(runs, items, n, src);
+}
+
+// Workgroups of the middle-stage kernel of a tile resident per CU.
+template <typename T, typename Tag>
+int stage_occupancy() {
+  constexpr int TX = Tag::tx, TY = Tag::ty;
+  const void* k;
+  if constexpr (Tag::kind == 2)
+    k = reinterpret_cast<const void*>(sph_stage_march_kernel<T, TX, TY, false, false, false>);
+  else if constexpr (Tag::kind == 1)
+    k = reinterpret_cast<const void*>(sph_stage_lds_kernel<T, TX, TY, false, false, false>);
+  else
+    k = reinterpret_cast<const void*>(sph_stage_kernel<T, TX, TY, false, false, false>);
+  const int threads = Tag::kind == 2 ? TX : TX * TY;
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, threads, 0);
+  return e == hipSuccess ? nb : 2048 / threads;
+}
 
 struct SphStateBase : SolverState {
   virtual int step(coffee_ctx* c, hipStream_t s) = 0;
@@ -85,37 +150,11 @@ struct SphState : SphStateBase {
     return COFFEE_OK;
   }
 
-  template <int TX, int TY, bool LDS>
-  static int occupancy() {
-    int nb = 0;
-    hipError_t e;
-    if (LDS)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void*>(sph_stage_lds_kernel<T, TX, TY, false, false, false>),
-          TX * TY, 0);
-    else
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void*>(sph_stage_kernel<T, TX, TY, false, false, false>),
-          TX * TY, 0);
-    return e == hipSuccess ? nb : 2048 / (TX * TY);
-  }
-
   int resident_per_cu() const {
-    switch (tile) {
-      case 0: return occupancy<64, 4, false>();
-      case 1: return occupancy<64, 8, false>();
-      case 2: return occupancy<128, 2, false>();
-      case 3: return occupancy<32, 8, false>();
-      case 4: return occupancy<256, 1, false>();
-      case 5: return occupancy<64, 8, true>();
-      case 6: return occupancy<64, 16, true>();
-      case 7: return occupancy<128, 4, true>();
-      case 8: return occupancy<32, 16, true>();
-      default: return occupancy<64, 4, true>();
-    }
+    return with_tile(tile, [](auto tag) { return stage_occupancy<T, decltype(tag)>(); });
   }
 
-  template <int TX, int TY, bool LDS>
+  template <typename Tag>
   int launch_step(coffee_ctx* c, hipStream_t s) {
     const int nr = (int)h_runs.size();
     const dim3 gb((max_n0 + 255) / 256, nr);
@@ -131,34 +170,25 @@ struct SphState : SphStateBase {
       const T as = (T)kAs[st], bs = (T)kBs[st], cs = (T)kCs[st];
       const char* name = st == 0 ? "sph_stage_first" : (st == 4 ? "sph_stage_last" : "sph_stage");
       rc = timed(c, name, s, [&] {
-#define SPH_LAUNCH(F, L, D)                                                                 \
-  do {                                                                                      \
-    if (LDS)                                                                                \
-      sph_stage_lds_kernel<T, TX, TY, F, L, D><<<grid_blocks, TX * TY, 0, s>>>(             \
-          d_runs, d_items, nitems, src, as, bs, cs);                                        \
-    else                                                                                    \
-      sph_stage_kernel<T, TX, TY, F, L, D><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, \
-                                                                          nitems, src, as,  \
-                                                                          bs, cs);          \
-  } while (0)
+        const int g = grid_blocks, n = nitems;
         if (st == 0) {
-          if (dg) SPH_LAUNCH(true, false, true); else SPH_LAUNCH(true, false, false);
+          if (dg) launch_stage<T, Tag, true, false, true>(g, s, d_runs, d_items, n, src, as, bs, cs);
+          else launch_stage<T, Tag, true, false, false>(g, s, d_runs, d_items, n, src, as, bs, cs);
         } else if (st == 4) {
-          if (dg) SPH_LAUNCH(false, true, true); else SPH_LAUNCH(false, true, false);
+          if (dg) launch_stage<T, Tag, false, true, true>(g, s, d_runs, d_items, n, src, as, bs, cs);
+          else launch_stage<T, Tag, false, true, false>(g, s, d_runs, d_items, n, src, as, bs, cs);
         } else {
-          if (dg) SPH_LAUNCH(false, false, true); else SPH_LAUNCH(false, false, false);
+          if (dg) launch_stage<T, Tag, false, false, true>(g, s, d_runs, d_items, n, src, as, bs, cs);
+          else launch_stage<T, Tag, false, false, false>(g, s, d_runs, d_items, n, src, as, bs, cs);
         }
-#undef SPH_LAUNCH
       });
       if (rc) return rc;
       rc = timed(c, "sph_axis", s,
                  [&] { sph_axis_kernel<T><<<gb, 256, 0, s>>>(d_runs, nr, src ^ 1); });
       if (rc) return rc;
     }
-    rc = timed(c, "sph_ko", s, [&] {
-      if (LDS) sph_ko_lds_kernel<T, TX, TY><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, nitems, 1);
-      else sph_ko_kernel<T, TX, TY><<<grid_blocks, TX * TY, 0, s>>>(d_runs, d_items, nitems, 1);
-    });
+    rc = timed(c, "sph_ko", s,
+               [&] { launch_ko<T, Tag>(grid_blocks, s, d_runs, d_items, nitems, 1); });
     if (rc) return rc;
     rc = timed(c, "sph_axis", s, [&] { sph_axis_kernel<T><<<gb, 256, 0, s>>>(d_runs, nr, 0); });
     if (rc) return rc;
@@ -168,18 +198,7 @@ struct SphState : SphStateBase {
   }
 
   int step_direct(coffee_ctx* c, hipStream_t s) {
-    switch (tile) {
-      case 0: return launch_step<64, 4, false>(c, s);
-      case 1: return launch_step<64, 8, false>(c, s);
-      case 2: return launch_step<128, 2, false>(c, s);
-      case 3: return launch_step<32, 8, false>(c, s);
-      case 4: return launch_step<256, 1, false>(c, s);
-      case 5: return launch_step<64, 8, true>(c, s);
-      case 6: return launch_step<64, 16, true>(c, s);
-      case 7: return launch_step<128, 4, true>(c, s);
-      case 8: return launch_step<32, 16, true>(c, s);
-      default: return launch_step<64, 4, true>(c, s);
-    }
+    return with_tile(tile, [&](auto tag) { return launch_step<decltype(tag)>(c, s); });
   }
 
   int step(coffee_ctx* c, hipStream_t s) override {
@@ -232,8 +251,9 @@ int validate_sph(coffee_ctx* c, const coffee_grid& g) {
   if (g.guard[1] < 4) return fail(c, COFFEE_ERR_INVALID, "theta guard must be >= 4 (axis mirror)");
   if (g.dims[0] - 2 * g.guard[0] < 1 || g.dims[1] - 2 * g.guard[1] < 4)
     return fail(c, COFFEE_ERR_INVALID, "grid too small");
-  if ((double)g.dims[0] * g.dims[1] >= 2147483647.0)
-    return fail(c, COFFEE_ERR_SIZE, "grid exceeds 2^31-1 cells per array");
+  // kernels address cells by 32-bit byte offsets
+  if ((double)g.dims[0] * g.dims[1] * c->scalar >= 4294967296.0)
+    return fail(c, COFFEE_ERR_SIZE, "grid exceeds 4 GiB per array");
   return COFFEE_OK;
 }
 
@@ -516,26 +536,12 @@ int autotune_impl(coffee_ctx* c, int trials, hipStream_t s, int* best, double* m
     int rc = S->build_items(c);
     if (rc) return rc;
     auto launch = [&] {
-      switch (t) {
-#define SPH_FIRST(TX, TY)                                                                 \
-  sph_stage_kernel<T, TX, TY, true, false, false><<<S->grid_blocks, TX * TY, 0, s>>>(      \
-      S->d_runs, S->d_items, S->nitems, 0, T(0), (T)kBs[0], T(0))
-#define SPH_FIRST_LDS(TX, TY)                                                             \
-  sph_stage_lds_kernel<T, TX, TY, true, false, false><<<S->grid_blocks, TX * TY, 0, s>>>(  \
-      S->d_runs, S->d_items, S->nitems, 0, T(0), (T)kBs[0], T(0))
-        case 0: SPH_FIRST(64, 4); break;
-        case 1: SPH_FIRST(64, 8); break;
-        case 2: SPH_FIRST(128, 2); break;
-        case 3: SPH_FIRST(32, 8); break;
-        case 4: SPH_FIRST(256, 1); break;
-        case 5: SPH_FIRST_LDS(64, 8); break;
-        case 6: SPH_FIRST_LDS(64, 16); break;
-        case 7: SPH_FIRST_LDS(128, 4); break;
-        case 8: SPH_FIRST_LDS(32, 16); break;
-        default: SPH_FIRST_LDS(64, 4); break;
-#undef SPH_FIRST
-#undef SPH_FIRST_LDS
-      }
+      with_tile(t, [&](auto tag) {
+        launch_stage<T, decltype(tag), true, false, false>(S->grid_blocks, s, S->d_runs,
+                                                           S->d_items, S->nitems, 0, T(0),
+                                                           (T)kBs[0], T(0));
+        return 0;
+      });
     };
     launch();
     HIP_TRY(c, hipEventRecord(a, s));

@@ -56,6 +56,37 @@ struct SphItem {
   int run, i0, j0;
 };
 
+// Global-memory access through run descriptors.  The field and table pointers
+// are read from the descriptor array, so the compiler cannot tell that they
+// address global memory and would emit flat instructions, which also count
+// against the LDS/scalar wait counter: every wait for an LDS read would then
+// wait for all outstanding HBM loads too, and a prefetch issued before a
+// barrier would be waited for at the barrier.  These cast to the global
+// address space.  gld_at / gst_at address cell `idx` of an array by a 32-bit
+// byte offset (every spherical array is < 4 GiB, checked when a run is set
+// up), so an access is a scalar base plus one vector offset shared by all
+// fields, without 64-bit address arithmetic per field.
+template <typename T>
+using GlobalCPtr = const __attribute__((address_space(1))) T*;
+template <typename T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(GlobalCPtr<T>)(p);
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(GlobalPtr<T>)(p) = v;
+}
+template <typename T>
+__device__ __forceinline__ T gld_at(const T* base, unsigned idx) {
+  return *(GlobalCPtr<T>)((const char*)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst_at(T* base, unsigned idx, T v) {
+  *(GlobalPtr<T>)((char*)base + idx * (unsigned)sizeof(T)) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ T sph_sqrtg(T r, T s) {
   // metric_sph.h:124-133: sqrt(g11 g22 g33), g11 = g22 = r^2, g33 = (r sin)^2
@@ -136,88 +167,116 @@ __device__ __forceinline__ void sph_pulsar(const SphRun<T>& R, T r, T th, T sth,
   }
 }
 
-template <int TX, int TY>
-__device__ __forceinline__ bool sph_cell(const SphItem& it, int dims0g, int& i, int& j) {
-  i = it.i0 + (int)(threadIdx.x % TX);
-  j = it.j0 + (int)(threadIdx.x / TX);
-  return true;
-}
-
-// One LSRK stage at interior cell (i, j) of run R.  F[f] points at the cell's
-// value of field f in the buffer read (global memory or an LDS tile) and sy
-// is that storage's row stride, so both kernels below run the same
-// arithmetic and give bitwise the same results.
-template <typename T, bool FIRST, bool LAST, bool DIAG>
-__device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j, int src, T As,
-                                                T Bs, T cs, const T* const* F, int sy) {
-  const int c = i + j * R.dims0;
-  const T* Ex = F[0];
-  const T* Ey = F[1];
-  const T* Ez = F[2];
-  const T* Bx = F[3];
-  const T* By = F[4];
-  const T* Bz = F[5];
-  const T* P = F[6];
-  // metric at the cell and its neighbours
-  T rr[5], ss[5];
-#pragma unroll
-  for (int q = 0; q < 5; q++) {
-    rr[q] = R.r[i - 2 + q];
-    ss[q] = R.sth[j - 2 + q];
+// Stencil operands of one cell.  x(f, d) is field f at radial offset d and
+// y(f, d) at theta offset d (|d| <= the kernel's halo).  StrideTaps reads
+// them from global memory or an LDS tile; QueueTaps takes the theta column
+// from a register queue and the radial neighbours from an LDS row.  Every
+// kernel below feeds the same per-cell arithmetic through one of these, so
+// all of them give bitwise the same results.
+template <typename T, bool GLOBAL>
+struct StrideTaps {
+  const T* const* F;  // F[f] = the cell's value of field f
+  int sy;             // row stride of that storage
+  __device__ __forceinline__ T x(int f, int d) const { return GLOBAL ? gld(F[f] + d) : F[f][d]; }
+  __device__ __forceinline__ T y(int f, int d) const {
+    return GLOBAL ? gld(F[f] + d * sy) : F[f][d * sy];
   }
-  const T r = rr[2], sth = ss[2];
+};
+
+template <typename T, int RH, int LW>
+struct QueueTaps {
+  const T (&q)[7][2 * RH + 1];  // q[f][RH + d] = field f at theta offset d
+  const T (&row)[7][LW];        // the cell's row, radial halo included
+  int lx;                       // the cell's slot in `row`
+  __device__ __forceinline__ T x(int f, int d) const { return d == 0 ? q[f][RH] : row[f][lx + d]; }
+  __device__ __forceinline__ T y(int f, int d) const { return q[f][RH + d]; }
+};
+
+// Metric tables around cell (i, j): r, r^3 at radial offsets -H..H, 1/r^2 at
+// the cell (stage only, H = 2), sin(theta) at theta offsets -H..H.
+template <typename T, int H>
+struct SphGeo {
+  T r[2 * H + 1], r3[2 * H + 1], s[2 * H + 1], ir2;
+  __device__ __forceinline__ void radial(const SphRun<T>& R, int i) {
+#pragma unroll
+    for (int q = 0; q <= 2 * H; q++) {
+      r[q] = gld(R.r + i - H + q);
+      r3[q] = gld(R.r3 + i - H + q);
+    }
+    ir2 = H == 2 ? gld(R.ir2 + i) : T(0);
+  }
+  __device__ __forceinline__ void polar(const SphRun<T>& R, int j) {
+#pragma unroll
+    for (int q = 0; q <= 2 * H; q++) s[q] = gld(R.sth + j - H + q);
+  }
+  // from a copy of the sin(theta) table, sl[d] = sin(theta_{j + d})
+  __device__ __forceinline__ void polar(const T* sl) {
+#pragma unroll
+    for (int q = 0; q <= 2 * H; q++) s[q] = sl[q - H];
+  }
+};
+
+// One LSRK stage at interior cell (i, j) of run R: S supplies the stencil
+// operands, G the metric tables, dprev the cell's LSRK register (ignored in
+// the first stage).
+template <typename T, bool FIRST, bool LAST, bool DIAG, typename Taps>
+__device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j, int src, T As,
+                                                T Bs, T cs, const Taps& S, const SphGeo<T, 2>& G,
+                                                const T (&dprev)[7]) {
+  const unsigned c = (unsigned)(i + j * R.dims0);
+  enum { EX, EY, EZ, BX, BY, BZ, PP };
+  const T r = G.r[2], sth = G.s[2];
   const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
   // sqrt(gamma) = r^3 |sin(theta)| from the per-run tables (metric_sph.h:124-133
   // takes the square root of g11 g22 g33, which is positive also in the guard
   // rows beyond the axis, where sin(theta) < 0)
-  T r3[5];
-#pragma unroll
-  for (int q = 0; q < 5; q++) r3[q] = R.r3[i - 2 + q];
-  T gm = r3[2] * fabs(sth);
+  T gm = G.r3[2] * fabs(sth);
   if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
   const T igm = T(1) / gm;
   // lowered components at x-neighbours (row j) and y-neighbours (column i)
-  auto dxl = [&](const T* f, int comp) {
+  auto dxl = [&](int f, int comp) {
     // d/dx of gamma_cc F (gamma depends on r only for c = 1, 2; on r and theta for c = 3)
     T v[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      const T rq = rr[q];
+      const T rq = G.r[q];
       const T gq = comp < 2 ? rq * rq : (rq * sth) * (rq * sth);
-      v[q] = q == 2 ? T(0) : gq * f[q - 2];
+      v[q] = q == 2 ? T(0) : gq * S.x(f, q - 2);
     }
     return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idx12;
   };
-  auto dyl = [&](const T* f, int comp) {
+  auto dyl = [&](int f, int comp) {
     T v[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      const T gq = comp < 2 ? r * r : (r * ss[q]) * (r * ss[q]);
-      v[q] = q == 2 ? T(0) : gq * f[(q - 2) * sy];
+      const T gq = comp < 2 ? r * r : (r * G.s[q]) * (r * G.s[q]);
+      v[q] = q == 2 ? T(0) : gq * S.y(f, q - 2);
     }
     return fma(T(8), v[3] - v[1], v[0] - v[4]) * R.idy12;
   };
-  const T rotBx = dyl(Bz, 2) * igm;
-  const T rotEx = dyl(Ez, 2) * igm;
-  const T rotBy = -dxl(Bz, 2) * igm;
-  const T rotBz = (dxl(By, 1) - dyl(Bx, 0)) * igm;
-  const T rotEy = -dxl(Ez, 2) * igm;
-  const T rotEz = (dxl(Ey, 1) - dyl(Ex, 0)) * igm;
+  const T rotBx = dyl(BZ, 2) * igm;
+  const T rotEx = dyl(EZ, 2) * igm;
+  const T rotBy = -dxl(BZ, 2) * igm;
+  const T rotBz = (dxl(BY, 1) - dyl(BX, 0)) * igm;
+  const T rotEy = -dxl(EZ, 2) * igm;
+  const T rotEz = (dxl(EY, 1) - dyl(EX, 0)) * igm;
   // div4_sph (.cu:99-124): d/dx(sqrt(g) F^x) + d/dy(sqrt(g) F^y), over sqrt(g)
-  auto divf = [&](const T* fx, const T* fy) {
+  auto divf = [&](int fx, int fy) {
     T vx[5], vy[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-      vx[q] = q == 2 ? T(0) : (r3[q] * fabs(sth)) * fx[q - 2];
-      vy[q] = q == 2 ? T(0) : (r3[2] * fabs(ss[q])) * fy[(q - 2) * sy];
+      vx[q] = q == 2 ? T(0) : (G.r3[q] * fabs(sth)) * S.x(fx, q - 2);
+      vy[q] = q == 2 ? T(0) : (G.r3[2] * fabs(G.s[q])) * S.y(fy, q - 2);
     }
     const T tx = fma(T(8), vx[3] - vx[1], vx[0] - vx[4]) * R.idx12;
     const T ty = fma(T(8), vy[3] - vy[1], vy[0] - vy[4]) * R.idy12;
     return (tx + ty) * igm;
   };
-  const T divE = divf(Ex, Ey);
-  const T divB = divf(Bx, By);
-  T u[7] = {Ex[0], Ey[0], Ez[0], Bx[0], By[0], Bz[0], P[0]};
+  const T divE = divf(EX, EY);
+  const T divB = divf(BX, BY);
+  T u[7];
+#pragma unroll
+  for (int f = 0; f < 7; f++) u[f] = S.x(f, 0);
   const T elx = g1 * u[0], ely = g2 * u[1], elz = g3 * u[2];
   const T blx = g1 * u[3], bly = g2 * u[4], blz = g3 * u[5];
   T B2 = u[3] * blx + u[4] * bly + u[5] * blz;
@@ -246,16 +305,16 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
   }
   T Px = 0, Py = 0;
   if (R.divB_clean) {
-    const T ig = R.ir2[i];
-    Px = fma(T(8), P[1] - P[-1], P[-2] - P[2]) * R.idx12 * ig;
-    Py = fma(T(8), P[sy] - P[-sy], P[-2 * sy] - P[2 * sy]) * R.idy12 * ig;
+    const T ig = G.ir2;
+    Px = fma(T(8), S.x(PP, 1) - S.x(PP, -1), S.x(PP, -2) - S.x(PP, 2)) * R.idx12 * ig;
+    Py = fma(T(8), S.y(PP, 1) - S.y(PP, -1), S.y(PP, -2) - S.y(PP, 2)) * R.idy12 * ig;
   }
   if (DIAG) {
-    R.J[0][c] = Jx;
-    R.J[1][c] = Jy;
-    R.J[2][c] = Jz;
-    R.divE[c] = divE;
-    R.divB[c] = divB;
+    gst_at(R.J[0], c, Jx);
+    gst_at(R.J[1], c, Jy);
+    gst_at(R.J[2], c, Jz);
+    gst_at(R.divE, c, divE);
+    gst_at(R.divB, c, divB);
   }
   const T dt = R.dt;
   T du[7];
@@ -268,16 +327,16 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
   du[6] = -dt * R.ch2 * divB - u[6] * R.inv_tau;
   if (!FIRST) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) du[q] = fma(As, R.dU[q][c], du[q]);
+    for (int q = 0; q < 7; q++) du[q] = fma(As, dprev[q], du[q]);
   }
   if (!LAST) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) R.dU[q][c] = du[q];
+    for (int q = 0; q < 7; q++) gst_at(R.dU[q], c, du[q]);
   }
   const T x = R.x0 + R.dx * T(i - R.g0);
   if (FIRST && R.pml_on && x > R.xh) {
 #pragma unroll
-    for (int q = 0; q < 7; q++) R.Un[q][c] = u[q];
+    for (int q = 0; q < 7; q++) gst_at(R.Un[q], c, u[q]);
   }
 #pragma unroll
   for (int q = 0; q < 7; q++) u[q] = fma(Bs, du[q], u[q]);
@@ -295,14 +354,21 @@ __device__ __forceinline__ void sph_stage_point(const SphRun<T>& R, int i, int j
       const T e = exp(-sig), f = (T(1) - e) / sig;
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        const T un = R.Un[q][c];
+        const T un = gld_at(R.Un[q], c);
         u[q] = e * un + f * (u[q] - un);
       }
     }
   }
   T* const* out = R.U[src ^ 1];
 #pragma unroll
-  for (int q = 0; q < 7; q++) out[q][c] = u[q];
+  for (int q = 0; q < 7; q++) gst_at(out[q], c, u[q]);
+}
+
+// The LSRK register of cell c (not read in the first stage).
+template <typename T, bool FIRST>
+__device__ __forceinline__ void sph_load_dprev(const SphRun<T>& R, unsigned c, T (&dp)[7]) {
+#pragma unroll
+  for (int q = 0; q < 7; q++) dp[q] = FIRST ? T(0) : gld_at(R.dU[q], c);
 }
 
 // One LSRK stage for every active run, stencil operands read from global
@@ -319,11 +385,17 @@ __global__ void __launch_bounds__(TX * TY)
     const int ii = it.i0 + li, jj = it.j0 + lj;  // interior-relative
     if (ii >= R.n0 || jj >= R.n1) continue;
     const int i = ii + R.g0, j = jj + R.g1;
-    const int c = i + j * R.dims0;
+    const unsigned c = (unsigned)(i + j * R.dims0);
     const T* F[7];
 #pragma unroll
     for (int f = 0; f < 7; f++) F[f] = R.U[src][f] + c;
-    sph_stage_point<T, FIRST, LAST, DIAG>(R, i, j, src, As, Bs, cs, F, R.dims0);
+    SphGeo<T, 2> G;
+    G.radial(R, i);
+    G.polar(R, j);
+    T dp[7];
+    sph_load_dprev<T, FIRST>(R, c, dp);
+    sph_stage_point<T, FIRST, LAST, DIAG>(R, i, j, src, As, Bs, cs, StrideTaps<T, true>{F, R.dims0}, G,
+                                          dp);
   }
 }
 
@@ -341,7 +413,7 @@ __device__ __forceinline__ void sph_fill_tile(const SphRun<T>& R, const SphItem&
     gj = gj > R.dims1 - 1 ? R.dims1 - 1 : gj;
     const int g = gi + gj * R.dims0;
 #pragma unroll
-    for (int f = 0; f < 7; f++) lds[f][idx] = R.U[src][f][g];
+    for (int f = 0; f < 7; f++) lds[f][idx] = gld(R.U[src][f] + g);
   }
 }
 
@@ -362,49 +434,191 @@ __global__ void __launch_bounds__(TX * TY)
     __syncthreads();
     const int ii = it.i0 + li, jj = it.j0 + lj;
     if (ii < R.n0 && jj < R.n1) {
+      const int i = ii + R.g0, j = jj + R.g1;
       const int slot = (lj + RH) * LXW + li + RH;
       const T* F[7];
 #pragma unroll
       for (int f = 0; f < 7; f++) F[f] = &lds[f][slot];
-      sph_stage_point<T, FIRST, LAST, DIAG>(R, ii + R.g0, jj + R.g1, src, As, Bs, cs, F, LXW);
+      SphGeo<T, 2> G;
+      G.radial(R, i);
+      G.polar(R, j);
+      T dp[7];
+      sph_load_dprev<T, FIRST>(R, (unsigned)(i + j * R.dims0), dp);
+      sph_stage_point<T, FIRST, LAST, DIAG>(R, i, j, src, As, Bs, cs, StrideTaps<T, false>{F, LXW}, G,
+                                            dp);
     }
     __syncthreads();
   }
 }
 
+// Theta-marching traversal shared by the march kernels.  A workgroup of TX
+// lanes owns a band of TX radial cells and walks up to NJ rows of it in
+// theta.  Each lane keeps the 2 RH + 1 theta rows of its column in a register
+// queue; the centre row, with the band's RH-cell radial halo fetched by the
+// edge lanes, goes through a double-buffered LDS row for the radial taps.
+// Loads are ordered so that no wait drains a prefetch (the vector memory
+// counter completes in order): per row, the halo of the next row, the cell's
+// own operands, then the row entering the queue two rows ahead (into one of
+// two alternating register sets); after the cell, the next row's LDS buffer
+// (centre and halo) is filled.  Every value is read from memory once per band.
+// `cell(q, row, lx, i, j, live, sl)` runs the per-cell work; lanes beyond the
+// run's last column (live = false) only help fill the rows.  Column and row
+// indices are clamped to the array; clamped values feed no result.
+template <typename T, int TX, int NJ, int RH, typename Cell>
+__device__ __forceinline__ void sph_march(const SphRun<T>& R, const SphItem& it, int src,
+                                          T (&lds)[2][7][TX + 2 * RH], T (&sl)[NJ + 2 * RH],
+                                          Cell&& cell) {
+  static_assert(TX > 2 * RH, "band narrower than its halo");
+  constexpr int NQ = 2 * RH + 1;
+  const int li = (int)threadIdx.x;
+  const int s = R.dims0, jmax = R.dims1 - 1;
+  const int ii = it.i0 + li;
+  const bool live = ii < R.n0;
+  const int gi = min(ii + R.g0, s - 1);
+  // edge lanes also fetch the radial halo: lanes 0..RH-1 the columns left of
+  // the band (slots 0..RH-1), lanes TX-RH..TX-1 those right of it
+  int hc = -1, hslot = 0;
+  if (li < RH) {
+    hc = it.i0 + R.g0 - RH + li;  // >= 0: the radial guard is >= RH
+    hslot = li;
+  } else if (li >= TX - RH) {
+    hc = min(it.i0 + R.g0 + li + RH, s - 1);
+    hslot = li + 2 * RH;
+  }
+  const T* const* U = R.U[src];
+  const int jc = it.j0 + R.g1;
+  const int nj = min(NJ, R.n1 - it.j0);
+  // sin(theta) of rows jc - RH .. jc + nj - 1 + RH (within the theta guards)
+  for (int t = li; t < nj + 2 * RH; t += TX) sl[t] = gld(R.sth + jc - RH + t);
+  auto load_row = [&](T(&v)[7], int j) {
+    const unsigned o = (unsigned)(gi + min(j, jmax) * s);
+#pragma unroll
+    for (int f = 0; f < 7; f++) v[f] = gld_at(U[f], o);
+  };
+  auto load_halo = [&](T(&v)[7], int j) {
+    if (hc >= 0) {
+      const unsigned o = (unsigned)(hc + min(j, jmax) * s);
+#pragma unroll
+      for (int f = 0; f < 7; f++) v[f] = gld_at(U[f], o);
+    }
+  };
+  auto put_row = [&](int buf, const T(&qq)[7][NQ], const T(&h)[7]) {
+#pragma unroll
+    for (int f = 0; f < 7; f++) {
+      lds[buf][f][li + RH] = qq[f][RH];
+      if (hc >= 0) lds[buf][f][hslot] = h[f];
+    }
+  };
+  T q[7][NQ], h[7] = {};
+  // queue rows jc - RH .. jc + RH; row jc (centre and halo) into buffer 0
+#pragma unroll
+  for (int d = 0; d < NQ; d++) {
+    const unsigned o = (unsigned)(gi + (jc - RH + d) * s);
+#pragma unroll
+    for (int f = 0; f < 7; f++) q[f][d] = gld_at(U[f], o);
+  }
+  load_halo(h, jc);
+  put_row(0, q, h);
+  T na[7], nb[7];
+  load_row(na, jc + RH + 1);
+  // row jc + k from buffer k & 1; `use` (row jc + k + RH + 1, loaded an
+  // iteration earlier) enters the queue afterwards, `pre` is loaded for the
+  // next iteration
+  auto body = [&](int k, T(&use)[7], T(&pre)[7]) {
+    const int j = jc + k;
+    load_halo(h, j + 1);
+    cell.prefetch(gi, j, live);
+    load_row(pre, j + RH + 2);
+    __syncthreads();
+    cell(q, lds[k & 1], li + RH, gi, j, live, &sl[k + RH]);
+#pragma unroll
+    for (int f = 0; f < 7; f++) {
+#pragma unroll
+      for (int d = 0; d < NQ - 1; d++) q[f][d] = q[f][d + 1];
+      q[f][NQ - 1] = use[f];
+    }
+    // buffer (k + 1) & 1 was last read in the previous iteration, which every
+    // lane finished before this iteration's barrier
+    put_row((k + 1) & 1, q, h);
+  };
+  for (int k = 0; k < nj; k += 2) {
+    body(k, na, nb);
+    if (k + 1 < nj) body(k + 1, nb, na);
+  }
+  // the next item's first row goes to buffer 0 (and its sin(theta) window to
+  // sl), which lanes may still read
+  __syncthreads();
+}
+
+// Per-cell work of the march kernels: prefetch() requests the cell's own
+// operands that are not stencil values (the LSRK register), operator() runs
+// the cell.
+template <typename T, bool FIRST, bool LAST, bool DIAG>
+struct SphStageCell {
+  static constexpr int RH = 2;
+  const SphRun<T>& R;
+  SphGeo<T, 2>& G;
+  int src;
+  T As, Bs, cs;
+  T dp[7];
+  __device__ __forceinline__ void prefetch(int i, int j, bool live) {
+    if (live) sph_load_dprev<T, FIRST>(R, (unsigned)(i + j * R.dims0), dp);
+  }
+  template <int LW>
+  __device__ __forceinline__ void operator()(const T (&q)[7][2 * RH + 1], const T (&row)[7][LW],
+                                             int lx, int i, int j, bool live, const T* sl) {
+    if (!live) return;
+    G.polar(sl);
+    sph_stage_point<T, FIRST, LAST, DIAG>(R, i, j, src, As, Bs, cs,
+                                          QueueTaps<T, RH, LW>{q, row, lx}, G, dp);
+  }
+};
+
+// The stage as a theta march (bands of TX radial cells, up to NJ rows).
+template <typename T, int TX, int NJ, bool FIRST, bool LAST, bool DIAG>
+__global__ void __launch_bounds__(TX)
+    sph_stage_march_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items,
+                           int nitems, int src, T As, T Bs, T cs) {
+  __shared__ T lds[2][7][TX + 4];
+  __shared__ T sl[NJ + 4];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const SphItem it = items[w];
+    const SphRun<T>& R = runs[it.run];
+    if (!R.active) continue;  // uniform: the whole workgroup works on one item
+    SphGeo<T, 2> G;
+    const int ii = it.i0 + (int)threadIdx.x;
+    if (ii < R.n0) G.radial(R, ii + R.g0);
+    SphStageCell<T, FIRST, LAST, DIAG> cell{R, G, src, As, Bs, cs, {}};
+    sph_march<T, TX, NJ, 2>(R, it, src, lds, sl, cell);
+  }
+}
+
 // Kreiss-Oliger on all 7 fields (kernel_KO_step1/2_sph, .cu:444-555):
 // KO_2d6 (sqrt(g)-weighted, KO_geometry) or KO_2d0, then projections and the
-// pulsar boundary at t + dt.  F, sy as in sph_stage_point.
-template <typename T>
+// pulsar boundary at t + dt.  S supplies the operands, G the metric tables.
+template <typename T, typename Taps>
 __device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, int src,
-                                             const T* const* F, int sy) {
-  const int c = i + j * R.dims0;
-  T rr[7], ss[7];
-#pragma unroll
-  for (int q = 0; q < 7; q++) {
-    rr[q] = R.r[i - 3 + q];
-    ss[q] = R.sth[j - 3 + q];
-  }
-  const T r = rr[3], sth = ss[3];
-  const T r3c = R.r3[i];
+                                             const Taps& S, const SphGeo<T, 3>& G) {
+  const unsigned c = (unsigned)(i + j * R.dims0);
+  const T r = G.r[3], sth = G.s[3];
+  const T r3c = G.r3[3];
   T gm = r3c * fabs(sth);
   if (gm < TinyOf<T>::v) gm = TinyOf<T>::v;
   T wx[7], wy[7];
 #pragma unroll
   for (int q = 0; q < 7; q++) {
-    wx[q] = R.KO_geometry ? R.r3[i - 3 + q] * fabs(sth) : T(1);
-    wy[q] = R.KO_geometry ? r3c * fabs(ss[q]) : T(1);
+    wx[q] = R.KO_geometry ? G.r3[q] * fabs(sth) : T(1);
+    wy[q] = R.KO_geometry ? r3c * fabs(G.s[q]) : T(1);
   }
   const T iden = R.KO_geometry ? T(1) / gm : T(1);
   T u[7];
 #pragma unroll
   for (int f = 0; f < 7; f++) {
-    const T* p = F[f];
-    const T kx = d6v(wx[0] * p[-3], wx[1] * p[-2], wx[2] * p[-1], wx[3] * p[0], wx[4] * p[1],
-                     wx[5] * p[2], wx[6] * p[3]);
-    const T ky = d6v(wy[0] * p[-3 * sy], wy[1] * p[-2 * sy], wy[2] * p[-sy], wy[3] * p[0],
-                     wy[4] * p[sy], wy[5] * p[2 * sy], wy[6] * p[3 * sy]);
-    u[f] = fma((kx + ky) * iden, R.KOfac, p[0]);
+    const T kx = d6v(wx[0] * S.x(f, -3), wx[1] * S.x(f, -2), wx[2] * S.x(f, -1),
+                     wx[3] * S.x(f, 0), wx[4] * S.x(f, 1), wx[5] * S.x(f, 2), wx[6] * S.x(f, 3));
+    const T ky = d6v(wy[0] * S.y(f, -3), wy[1] * S.y(f, -2), wy[2] * S.y(f, -1),
+                     wy[3] * S.x(f, 0), wy[4] * S.y(f, 1), wy[5] * S.y(f, 2), wy[6] * S.y(f, 3));
+    u[f] = fma((kx + ky) * iden, R.KOfac, S.x(f, 0));
   }
   const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
   sph_project(R, g1, g2, g3, u);
@@ -412,7 +626,7 @@ __device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, i
   sph_pulsar(R, r, th, sth, g1, g2, g3, (T)R.t + R.dt, u);
   T* const* out = R.U[src ^ 1];
 #pragma unroll
-  for (int q = 0; q < 7; q++) out[q][c] = u[q];
+  for (int q = 0; q < 7; q++) gst_at(out[q], c, u[q]);
 }
 
 template <typename T, int TX, int TY>
@@ -426,11 +640,14 @@ __global__ void __launch_bounds__(TX * TY)
     const int ii = it.i0 + (int)(threadIdx.x % TX), jj = it.j0 + (int)(threadIdx.x / TX);
     if (ii >= R.n0 || jj >= R.n1) continue;
     const int i = ii + R.g0, j = jj + R.g1;
-    const int c = i + j * R.dims0;
+    const unsigned c = (unsigned)(i + j * R.dims0);
     const T* F[7];
 #pragma unroll
     for (int f = 0; f < 7; f++) F[f] = R.U[src][f] + c;
-    sph_ko_point<T>(R, i, j, src, F, R.dims0);
+    SphGeo<T, 3> G;
+    G.radial(R, i);
+    G.polar(R, j);
+    sph_ko_point<T>(R, i, j, src, StrideTaps<T, true>{F, R.dims0}, G);
   }
 }
 
@@ -449,13 +666,52 @@ __global__ void __launch_bounds__(TX * TY)
     __syncthreads();
     const int ii = it.i0 + li, jj = it.j0 + lj;
     if (ii < R.n0 && jj < R.n1) {
+      const int i = ii + R.g0, j = jj + R.g1;
       const int slot = (lj + RH) * LXW + li + RH;
       const T* F[7];
 #pragma unroll
       for (int f = 0; f < 7; f++) F[f] = &lds[f][slot];
-      sph_ko_point<T>(R, ii + R.g0, jj + R.g1, src, F, LXW);
+      SphGeo<T, 3> G;
+      G.radial(R, i);
+      G.polar(R, j);
+      sph_ko_point<T>(R, i, j, src, StrideTaps<T, false>{F, LXW}, G);
     }
     __syncthreads();
+  }
+}
+
+template <typename T>
+struct SphKoCell {
+  static constexpr int RH = 3;
+  const SphRun<T>& R;
+  SphGeo<T, 3>& G;
+  int src;
+  __device__ __forceinline__ void prefetch(int, int, bool) {}
+  template <int LW>
+  __device__ __forceinline__ void operator()(const T (&q)[7][2 * RH + 1], const T (&row)[7][LW],
+                                             int lx, int i, int j, bool live, const T* sl) {
+    if (!live) return;
+    G.polar(sl);
+    sph_ko_point<T>(R, i, j, src, QueueTaps<T, RH, LW>{q, row, lx}, G);
+  }
+};
+
+// Kreiss-Oliger as a theta march (as the stage).
+template <typename T, int TX, int NJ>
+__global__ void __launch_bounds__(TX)
+    sph_ko_march_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items,
+                        int nitems, int src) {
+  __shared__ T lds[2][7][TX + 6];
+  __shared__ T sl[NJ + 6];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const SphItem it = items[w];
+    const SphRun<T>& R = runs[it.run];
+    if (!R.active) continue;  // uniform: the whole workgroup works on one item
+    SphGeo<T, 3> G;
+    const int ii = it.i0 + (int)threadIdx.x;
+    if (ii < R.n0) G.radial(R, ii + R.g0);
+    SphKoCell<T> cell{R, G, src};
+    sph_march<T, TX, NJ, 3>(R, it, src, lds, sl, cell);
   }
 }
 
@@ -472,44 +728,47 @@ __global__ void __launch_bounds__(256)
     const int i = ii + R.g0;
     const int s = R.dims0;
     T* const* U = R.U[buf];
+    // U[f][dst] = sign * U[f][src]
+    auto mv = [&](int f, int dst, int src, T sign) { gst(U[f] + dst, sign * gld(U[f] + src)); };
+    auto zero = [&](int f, int dst) { gst(U[f] + dst, T(0)); };
     int j = R.g1;
     int c = i + j * s;
     T y = R.y0 + R.dy * T(j - R.g1);
     if (fabs(y) < R.dy / T(2)) {
-      U[0][c] = U[0][c + s];
-      U[1][c] = T(0);
-      U[2][c] = T(0);
-      U[3][c] = U[3][c + s];
-      U[4][c] = T(0);
-      U[5][c] = T(0);
-      U[6][c] = U[6][c + s];
+      mv(0, c, c + s, T(1));
+      zero(1, c);
+      zero(2, c);
+      mv(3, c, c + s, T(1));
+      zero(4, c);
+      zero(5, c);
+      mv(6, c, c + s, T(1));
       for (int l = 1; l <= 3; ++l) {
-        U[0][c - l * s] = U[0][c + l * s];
-        U[1][c - l * s] = -U[1][c + l * s];
-        U[2][c - l * s] = -U[2][c + l * s];
-        U[3][c - l * s] = U[3][c + l * s];
-        U[4][c - l * s] = -U[4][c + l * s];
-        U[5][c - l * s] = -U[5][c + l * s];
+        mv(0, c - l * s, c + l * s, T(1));
+        mv(1, c - l * s, c + l * s, T(-1));
+        mv(2, c - l * s, c + l * s, T(-1));
+        mv(3, c - l * s, c + l * s, T(1));
+        mv(4, c - l * s, c + l * s, T(-1));
+        mv(5, c - l * s, c + l * s, T(-1));
       }
     }
     j = R.dims1 - R.g1;
     c = i + j * s;
     y = R.y0 + R.dy * T(j - R.g1);
     if (fabs(y - T(M_PI)) < R.dy / T(2)) {
-      U[0][c] = U[0][c - s];
-      U[1][c] = T(0);
-      U[2][c] = T(0);
-      U[3][c] = U[3][c - s];
-      U[4][c] = T(0);
-      U[5][c] = T(0);
-      U[6][c] = U[6][c - s];
+      mv(0, c, c - s, T(1));
+      zero(1, c);
+      zero(2, c);
+      mv(3, c, c - s, T(1));
+      zero(4, c);
+      zero(5, c);
+      mv(6, c, c - s, T(1));
       for (int l = 1; l <= 3; ++l) {
-        U[0][c + l * s] = U[0][c - l * s];
-        U[1][c + l * s] = -U[1][c - l * s];
-        U[2][c + l * s] = -U[2][c - l * s];
-        U[3][c + l * s] = U[3][c - l * s];
-        U[4][c + l * s] = -U[4][c - l * s];
-        U[5][c + l * s] = -U[5][c - l * s];
+        mv(0, c + l * s, c - l * s, T(1));
+        mv(1, c + l * s, c - l * s, T(-1));
+        mv(2, c + l * s, c - l * s, T(-1));
+        mv(3, c + l * s, c - l * s, T(1));
+        mv(4, c + l * s, c - l * s, T(-1));
+        mv(5, c + l * s, c - l * s, T(-1));
       }
     }
   }
@@ -532,7 +791,7 @@ __global__ void __launch_bounds__(256)
     const bool guard = i < R.g0 || i >= R.dims0 - R.g0 || j < R.g1 || j >= R.dims1 - R.g1;
     if (!guard) continue;
 #pragma unroll
-    for (int q = 0; q < 7; q++) R.U[1][q][t] = R.U[0][q][t];
+    for (int q = 0; q < 7; q++) gst(R.U[1][q] + t, gld(R.U[0][q] + t));
   }
 }
 

@@ -186,8 +186,10 @@ int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid *grid
 int coffee_sph_batch_destroy(coffee_batch *b);
 /* Advance every active run by `nsteps` steps (asynchronous on `stream`). */
 int coffee_sph_batch_step(coffee_batch *b, int nsteps, void *stream);
-/* Tile shapes the kernels are built for: coffee_sph_num_tiles() of them,
- * tile i being tx x ty cells (threads) per work item. */
+/* Tile shapes the kernels are built for: coffee_sph_num_tiles() of them.
+ * Tile i has tx x ty threads, one cell each, or, for the theta-marching
+ * tiles, tx threads that walk a band of tx radial cells over ty theta rows.
+ * Every tile gives bitwise the same results. */
 int coffee_sph_num_tiles(void);
 int coffee_sph_tile_shape(int tile_index, int *tx, int *ty);
 int coffee_sph_batch_set_tile(coffee_batch *b, int tile_index);
