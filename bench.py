@@ -57,6 +57,8 @@ def parse():
                     help="alfven3d: the headline flat 512^3 run (default); sph-sweep: batched 2D "
                          "pulsar runs; gr-wald: Kerr-Schild Wald problem")
     ap.add_argument("--runs", type=int, default=256, help="sph-sweep: number of runs")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="sph-sweep: tile index (default: autotuned)")
     ap.add_argument("--spin", type=float, default=0.9, help="gr-wald: black-hole spin a")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a plain device copy of this many MiB (3 launches), for "
@@ -191,7 +193,11 @@ def run_sph_sweep(args):
         cells += nr * nth
         rlcs.append(rlc)
     b = SphBatch(envs, datas, nsteps=[0] * args.runs)
-    best, tile_ms = b.autotune(trials=3)
+    if args.tile >= 0:
+        b.set_tile(args.tile)
+        best, tile_ms = args.tile, []
+    else:
+        best, tile_ms = b.autotune(trials=3)
     b.use_graph(True)
     print(f"[bench] sph-sweep {args.runs} runs, {cells / 1e6:.1f} M cells, set-up "
           f"{time.perf_counter() - t0:.1f}s, tile {best} {tile_ms}", file=sys.stderr, flush=True)

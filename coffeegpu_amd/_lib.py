@@ -121,6 +121,7 @@ SIGNATURES = {
                                         _I, _I, _V]),
     "coffee_unet_unpack": (ctypes.c_int, [_V, _I, _I, _V * 6, _I, _I, _I, _I, _I, _I, _D, _D, _D,
                                           _D, _V]),
+    "coffee_unet_corotate": (ctypes.c_int, [_V * 6, _I, _I, _I, _I, _I, _I, _D, _D, _D, _D, _V]),
     "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_void_p]),
     "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,

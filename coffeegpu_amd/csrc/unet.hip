@@ -371,6 +371,24 @@ __global__ void unpack_kernel(const float* in, int cst, long base, T* f0, T* f1,
   }
 }
 
+// Corotation: E = -(Omega x r) x B with v = Omega r sin(theta) (phi direction),
+// in upper components of x = ln r, theta: E^x = v B^theta, E^theta = -v B^x,
+// E^phi = 0 (boundary_pulsar_sph applies the same relation inside the star).
+template <typename T>
+__global__ void corotate_kernel(T* ex, T* ey, T* ez, const T* bx, const T* by, int dims0, int g0,
+                                int g1, int n0, int n1, double x0, double dx, double dy,
+                                double omega) {
+  const long total = (long)n0 * n1;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % n0), j = (int)(t / n0);
+    const long c = (long)(i + g0) + (long)(j + g1) * dims0;
+    const double v = omega * exp(x0 + dx * i) * sin(dy * j);
+    ex[c] = (T)(v * (double)by[c]);
+    ey[c] = (T)(-v * (double)bx[c]);
+    ez[c] = T(0);
+  }
+}
+
 int grid_for(long total, int block = 256) {
   long b = (total + block - 1) / block;
   return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
@@ -563,6 +581,29 @@ int coffee_unet_unpack(const float* in, int cst, int image, void* const fields[6
                                                  dims0, g0, g1, n0, n1, x0, dx, dy, rpow);
   }
   return launch_check("unpack");
+}
+
+int coffee_unet_corotate(void* const fields[6], int scalar_bytes, int dims0, int g0, int g1,
+                         int n0, int n1, double x0, double dx, double dy, double omega,
+                         void* stream) {
+  if (!fields || n0 < 1 || n1 < 1 || dims0 < n0 + 2 * g0 || g0 < 0 || g1 < 0)
+    return fail(nullptr, COFFEE_ERR_INVALID, "corotate: bad arguments");
+  for (int q = 0; q < 5; q++)
+    if (!fields[q]) return fail(nullptr, COFFEE_ERR_INVALID, "corotate: null field");
+  const int blocks = grid_for((long)n0 * n1);
+  hipStream_t s = (hipStream_t)stream;
+  if (scalar_bytes == 4) {
+    float* const* f = (float* const*)fields;
+    corotate_kernel<float><<<blocks, 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], dims0, g0, g1, n0,
+                                                  n1, x0, dx, dy, omega);
+  } else if (scalar_bytes == 8) {
+    double* const* f = (double* const*)fields;
+    corotate_kernel<double><<<blocks, 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], dims0, g0, g1,
+                                                   n0, n1, x0, dx, dy, omega);
+  } else {
+    return fail(nullptr, COFFEE_ERR_INVALID, "corotate: scalar_bytes must be 4 or 8");
+  }
+  return launch_check("corotate");
 }
 
 }  // extern "C"

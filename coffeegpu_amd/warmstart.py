@@ -124,6 +124,20 @@ def unpack_runs(runs: Runs, src, cst: int):
                                         n1, g.lower[0], g.delta[0], g.delta[1], RPOW, s))
 
 
+def corotate_runs(runs: Runs):
+    """Replace E by the corotating field of B, E = -(Omega x r) x B, in every
+    run's interior (the steady aligned rotator satisfies it on every field
+    line, so a predicted B fixes E)."""
+    torch = _torch()
+    L = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for env, d in zip(runs.envs, runs.datas):
+        g, n0, n1 = _geom(env)
+        _lib.check(L.coffee_unet_corotate(_field_ptrs(d), env.dtype.itemsize, g.dims[0],
+                                          g.guard[0], g.guard[1], n0, n1, g.lower[0], g.delta[0],
+                                          g.delta[1], env.params().omega, s))
+
+
 def relax(runs: Runs, chunk: int = 500, log=None):
     """Advance every run to its own step target."""
     torch = _torch()
@@ -200,7 +214,7 @@ def train(net: UNet, X, T, n: int, h: int, w: int, epochs: int, batch: int, lr: 
 
 def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0, rlc_hi=20.0,
                levels=4, base=32, epochs=60, batch=8, lr=1e-3, tol=0.01, seed=0, log=None,
-               eval_periods=None):
+               eval_periods=None, corotate=False):
     torch = _torch()
     t_start = time.perf_counter()
     rng = np.random.default_rng(seed)
@@ -257,6 +271,8 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
         o = (out.view(m * H * W, pad16(cout))[:, :cout] * chan_rms).reshape(-1)
         pred[b * H * W * cout:(b + m) * H * W * cout] = o + base0[b * H * W * cout:(b + m) * H * W * cout]
     unpack_runs(warm, pred, cout)
+    if corotate:
+        corotate_runs(warm)
     h_dip = luminosity_history(dip, check, steps)
     h_warm = luminosity_history(warm, check, steps)
     # field error of the prediction vs the relaxed (dipole-run) state, in network units
@@ -299,7 +315,7 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
                            "warm_settled": sum(1 for r in res if r["settle_by_tol"][tt][1])}
     return {
         "grid": [nr, nth], "n_train": n_train, "n_test": n_test, "periods": periods,
-        "eval_periods": ep, "tol": tol,
+        "eval_periods": ep, "tol": tol, "corotate": corotate,
         "rlc_range": [rlc_lo, rlc_hi], "levels": levels, "base": base,
         "params": net.num_params(), "epochs": epochs, "loss_first": losses[0],
         "loss_last": losses[-1], "relax_s": t_relax, "relax_cell_updates_per_s": cells / t_relax,
@@ -326,6 +342,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-periods", type=float, default=0.0)
+    ap.add_argument("--corotate", action="store_true",
+                    help="set the warm start's E from its B by corotation")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
 
@@ -334,7 +352,7 @@ def main(argv=None):
 
     out = experiment(a.nr, a.nth, a.train, a.test, a.periods, a.rlc[0], a.rlc[1], a.levels, a.base,
                      a.epochs, a.batch, a.lr, seed=a.seed, log=log,
-                     eval_periods=a.eval_periods or None)
+                     eval_periods=a.eval_periods or None, corotate=a.corotate)
     print(json.dumps(out))
 
 

@@ -86,6 +86,14 @@ int coffee_unet_pack(void *const fields[6], int scalar_bytes, int dims0, int g0,
 int coffee_unet_unpack(const float *in, int cst, int image, void *const fields[6],
                        int scalar_bytes, int dims0, int g0, int g1, int n0, int n1, double x0,
                        double dx, double dy, double rpow, void *stream);
+/* Corotating electric field from B on a run's interior: E = -(Omega x r) x B,
+ * i.e. E^x = v B^theta, E^theta = -v B^x, E^phi = 0 with v = omega r sin(theta)
+ * (the relation boundary_pulsar_sph imposes inside the star, and which the
+ * steady aligned rotator satisfies on every field line).  Geometry as in
+ * coffee_unet_unpack; fields[5] is not read. */
+int coffee_unet_corotate(void *const fields[6], int scalar_bytes, int dims0, int g0, int g1,
+                         int n0, int n1, double x0, double dx, double dy, double omega,
+                         void *stream);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ torch = pytest.importorskip("torch")
 if not torch.cuda.is_available():  # pragma: no cover
     pytest.skip("no GPU", allow_module_level=True)
 
-from coffeegpu_amd.warmstart import experiment, settle_step  # noqa: E402
+from coffeegpu_amd.warmstart import Runs, corotate_runs, experiment, settle_step  # noqa: E402
 
 
 def test_settle_step():
@@ -29,3 +29,39 @@ def test_small_warm_start_experiment():
     assert out["loss_last"] < 0.5 * out["loss_first"]
     assert out["median_field_err_warm"] < out["median_field_err_dipole"]
     assert all(np.isfinite(r["L_ref"]) and r["L_ref"] > 0 for r in out["runs"])
+
+
+def test_corotate_matches_pulsar_boundary():
+    """The corotation kernel reproduces, inside the star, the electric field the
+    solver's pulsar boundary (boundary_pulsar_sph) imposes from the dipole, and
+    leaves E perpendicular to B everywhere."""
+    runs = Runs([4.0], 64, 32, [1])
+    runs.batch.step(1)
+    torch.cuda.synchronize()
+    env, d = runs.envs[0], runs.datas[0]
+    g = env.grid()
+    E_solver = [a.sync_to_host().copy() for a in d.state_arrays()[:3]]
+    corotate_runs(runs)
+    torch.cuda.synchronize()
+    st = [a.sync_to_host() for a in d.state_arrays()[:6]]
+    runs.close()
+    sp = env.params()
+    x, y, _ = (np.array(v, dtype=np.float64) for v in g.coords())
+    r = np.exp(x) + 0 * y  # (1, dims1, dims0)
+    it = g.interior()
+    # strictly inside the star, where the boundary overwrites all fields
+    star = (r < sp.radius - 1e-3) & (np.abs(r - sp.radius) >= 0.5 * g.delta[0])
+    mask = np.zeros(g.shape, bool)
+    mask[it] = star[it]
+    assert mask.sum() > 10
+    scale = max(np.abs(E_solver[0][mask]).max(), np.abs(E_solver[1][mask]).max())
+    for c in range(3):
+        assert np.abs(st[c][mask] - E_solver[c][mask]).max() <= 1e-5 * scale, c
+    # E.B = r^2 (E^x B^x + E^th B^th) (E^phi = 0) vanishes on the whole interior
+    sel = np.zeros(g.shape, bool)
+    sel[it] = True
+    e0, e1, b0, b1 = (a.astype(np.float64) for a in (st[0], st[1], st[3], st[4]))
+    EB = e0 * b0 + e1 * b1
+    mag = np.sqrt((e0 ** 2 + e1 ** 2) * (b0 ** 2 + b1 ** 2))
+    assert np.all(np.abs(EB[sel]) <= 1e-6 * mag[sel] + 1e-30)
+    assert np.all(st[2][sel] == 0)
