@@ -49,7 +49,9 @@ def parse():
     ap.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variant", type=int, default=0, help="0 = LDS-tiled (default), 1 = simple")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="0 = LDS-tiled (default tile), 1 = simple, 2-4 = 64-wide tiles of "
+                         "4/8/16 rows, 5/6 = 128/256-wide tiles, 7 = 128 x 4")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0 = auto)")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=10)
@@ -222,6 +224,11 @@ def run_sph_sweep(args):
     bytes_per_launch = SPH_SCALARS["sph_stage"] * 4 * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
     lum = b.luminosity(1.5 * np.asarray(rlcs))
+    # HBM bytes per middle-stage launch, measured for one tile and batch size
+    tr = load_traffic("sph_sweep_fp32_traffic.json") or {}
+    traffic = tr.get("bytes_per_launch") if (tr.get("tile") == best and
+                                             tr.get("runs") == args.runs and
+                                             tr.get("seed") == args.seed) else None
     out = {
         "metric": "cell-updates/s (batched 2D log-spherical pulsar sweep, 1 GPU)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -232,7 +239,7 @@ def run_sph_sweep(args):
                                "r_out 60, absorbing r>48, hipGraph replay, autotuned tile",
                    "cells": cells, "tile": best, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "sph_stage",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "sph_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "step_algorithmic_GBps": SCALARS_PER_CELL_UPDATE * 4 * value / 1e9,
         "kernels": _kernel_table(timing, SPH_SCALARS, 4, cells),

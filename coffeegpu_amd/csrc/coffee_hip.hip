@@ -170,41 +170,58 @@ int auto_zchunk(int tiles_per_plane, int nz) {
   return zc < 8 ? 8 : zc;
 }
 
-// Rows per tile of the tiled kernels: variant 0 picks the measured best for
-// the Scalar type on MI355X at 512^3 (FP32 16 rows = 1024 threads, FP64 8
-// rows = 512 threads); 2/3/4 force 4/8/16 rows (16 only in FP32); variant 1
-// is the thread-per-cell schedule.
+// Tile (columns x rows) of the tiled kernels: variant 0 picks the measured
+// best for the Scalar type on MI355X at 512^3 (FP32 128 x 8 = 1024 threads,
+// 3.5 % faster per step than 64 x 16; FP64 64 x 8 = 512 threads, equal to
+// 128 x 4); 2/3/4 force 64 x 4/8/16 (16 only in FP32), 5 and 6 wider rows
+// (FP32 128 x 8, 256 x 4; FP64 128 x 4 for both: a 256 x 2 FP64 tile does not
+// fit in LDS), 7 = 128 x 4; variant 1 is the thread-per-cell schedule.
 template <typename T>
-int tile_rows(int variant) {
-  if (variant == 2) return 4;
-  if (variant == 3) return 8;
-  if (variant == 4) return sizeof(T) == 4 ? 16 : 8;
-  return sizeof(T) == 4 ? 16 : 8;
+void tile_shape(int variant, int& tx, int& ty) {
+  const bool f32 = sizeof(T) == 4;
+  tx = f32 ? 128 : 64;
+  ty = 8;
+  if (variant == 2 || variant == 3 || variant == 4) {
+    tx = 64;
+    ty = variant == 2 ? 4 : (variant == 3 || !f32 ? 8 : 16);
+  }
+  if (variant == 5) {
+    tx = 128;
+    ty = f32 ? 8 : 4;
+  }
+  if (variant == 6) {
+    tx = f32 ? 256 : 128;
+    ty = 4;
+  }
+  if (variant == 7) {
+    tx = 128;
+    ty = 4;
+  }
 }
 
-template <typename T, bool F, bool L, bool D, bool S, int TY>
+template <typename T, bool F, bool L, bool D, bool S, int TX, int TY>
 void launch_stage(bool tiled, int tblocks, dim3 grid, dim3 block, hipStream_t s, const FlatK<T>& k,
                   const FlatIO<T>& io, T as, T bs, T ts, const TileMap& tm) {
   if (tiled)
-    ez_stage_tiled<T, TY, F, L, D, S><<<tblocks, 64 * TY, 0, s>>>(k, io, as, bs, ts, tm);
+    ez_stage_tiled<T, TX, TY, F, L, D, S><<<tblocks, TX * TY, 0, s>>>(k, io, as, bs, ts, tm);
   else
     ez_stage_kernel<T, F, L, D, S><<<grid, block, 0, s>>>(k, io, as, bs, ts);
 }
 
-template <typename T, bool F, bool L, int TY>
+template <typename T, bool F, bool L, int TX, int TY>
 void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim3 block,
                      hipStream_t s, const FlatK<T>& k, const FlatIO<T>& io, T as, T bs, T ts,
                      const TileMap& tm) {
   if (dg) {
-    if (star) launch_stage<T, F, L, true, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-    else launch_stage<T, F, L, true, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    if (star) launch_stage<T, F, L, true, true, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, true, false, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
   } else {
-    if (star) launch_stage<T, F, L, false, true, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
-    else launch_stage<T, F, L, false, false, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    if (star) launch_stage<T, F, L, false, true, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+    else launch_stage<T, F, L, false, false, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
   }
 }
 
-template <typename T, int TY>
+template <typename T, int TX, int TY>
 int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7], T* const B[7],
                   double time, hipStream_t s, bool star) {
   static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
@@ -216,7 +233,7 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
   dim3 block(64, 4, 1);
   dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
   TileMap tm;
-  tm.ntx = (k.n[0] + 63) / 64;
+  tm.ntx = (k.n[0] + TX - 1) / TX;
   tm.nty = (k.n[1] + TY - 1) / TY;
   tm.zc = c->zchunk > 0 ? c->zchunk : auto_zchunk(tm.ntx * tm.nty, k.n[2]);
   if (tm.zc > k.n[2]) tm.zc = k.n[2];
@@ -239,11 +256,11 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
     const char* name = st == 0 ? "ez_stage_first" : (st == 4 ? "ez_stage_last" : "ez_stage");
     rc = timed(c, name, s, [&] {
       if (st == 0)
-        launch_stage_fl<T, true, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+        launch_stage_fl<T, true, false, TX, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
       else if (st == 4)
-        launch_stage_fl<T, false, true, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+        launch_stage_fl<T, false, true, TX, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
       else
-        launch_stage_fl<T, false, false, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
+        launch_stage_fl<T, false, false, TX, TY>(dg, star, tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
     });
     if (rc) return rc;
   }
@@ -255,8 +272,8 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
   const T t1 = t0 + dt;
   return timed(c, "ez_ko", s, [&] {
     if (tiled) {
-      if (star) ez_ko_tiled<T, TY, true><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
-      else ez_ko_tiled<T, TY, false><<<tblocks, 64 * TY, 0, s>>>(k, io, t1, tm);
+      if (star) ez_ko_tiled<T, TX, TY, true><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
+      else ez_ko_tiled<T, TX, TY, false><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
     } else {
       if (star) ez_ko_kernel<T, true><<<grid, block, 0, s>>>(k, io, t1);
       else ez_ko_kernel<T, false><<<grid, block, 0, s>>>(k, io, t1);
@@ -310,10 +327,15 @@ int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t
   io.divB = (T*)f->divB;
 
   const bool star = k.problem == 1 || k.problem == 2;
-  const int ty = tile_rows<T>(c->variant);
-  if (ty == 4) rc = ez_launch_all<T, 4>(c, k, io, A, B, time, s, star);
-  else if (ty == 16) rc = ez_launch_all<T, 16>(c, k, io, A, B, time, s, star);
-  else rc = ez_launch_all<T, 8>(c, k, io, A, B, time, s, star);
+  int tx = 64, ty = 8;
+  tile_shape<T>(c->variant, tx, ty);
+  constexpr bool f32 = sizeof(T) == 4;
+  if (tx == 128 && ty == 4) rc = ez_launch_all<T, 128, 4>(c, k, io, A, B, time, s, star);
+  else if (tx == 128) rc = ez_launch_all<T, 128, f32 ? 8 : 4>(c, k, io, A, B, time, s, star);
+  else if (tx == 256) rc = ez_launch_all<T, f32 ? 256 : 128, 4>(c, k, io, A, B, time, s, star);
+  else if (ty == 4) rc = ez_launch_all<T, 64, 4>(c, k, io, A, B, time, s, star);
+  else if (ty == 16) rc = ez_launch_all<T, 64, 16>(c, k, io, A, B, time, s, star);
+  else rc = ez_launch_all<T, 64, 8>(c, k, io, A, B, time, s, star);
   if (rc) return rc;
   // single-GPU exchange: periodic guards of the caller's arrays
   return guard_slabs<T>(c, s, A, A, true);
@@ -565,7 +587,7 @@ int coffee_fill_guards_arrays(const coffee_grid* grid, const int periodic[3], vo
 
 int coffee_set_schedule(coffee_ctx* c, int variant, int zchunk) {
   if (!c) return fail(c, COFFEE_ERR_INVALID, "null argument");
-  if (variant < 0 || variant > 4) return fail(c, COFFEE_ERR_INVALID, "variant must be 0..4");
+  if (variant < 0 || variant > 7) return fail(c, COFFEE_ERR_INVALID, "variant must be 0..7");
   if (zchunk < 0) return fail(c, COFFEE_ERR_INVALID, "zchunk must be >= 0");
   c->variant = variant;
   c->zchunk = zchunk;

@@ -1,6 +1,6 @@
 // LDS-tiled, z-marching kernels of the flat EZ path (gfx950).
 //
-// A workgroup owns a 64 x TY tile of the x-y plane and marches it through a
+// A workgroup owns a TX x TY tile of the x-y plane and marches it through a
 // chunk of z planes.  Per plane, every field of the tile plus an R-cell halo
 // in x and y is staged in LDS (double-buffered, one barrier per plane); x and y
 // derivatives read LDS, z derivatives read a per-thread register queue of the
@@ -10,8 +10,8 @@
 // extra planes per chunk).  Halo values of the next plane are prefetched into
 // registers while the current plane is computed.
 //
-// Rows of 64 cells map to one wavefront, so all plane loads and stores are
-// fully coalesced 256-byte (FP32) rows.  Tiles are assigned XCD-contiguously:
+// Rows of TX = 64, 128 or 256 cells map to one to four wavefronts, so all
+// plane loads and stores are fully coalesced rows.  Tiles are assigned XCD-contiguously:
 // workgroups are dispatched round-robin over the 8 XCDs, and the remap gives
 // each XCD a contiguous range of tiles so that halo rows hit its own L2.
 //
@@ -65,11 +65,11 @@ __device__ __forceinline__ void halo_xy(int h, int& hx, int& hy) {
   }
 }
 
-template <typename T, int TY, bool FIRST, bool LAST, bool DIAG, bool STAR>
-__global__ void __launch_bounds__(64 * TY)
+template <typename T, int TX, int TY, bool FIRST, bool LAST, bool DIAG, bool STAR>
+__global__ void __launch_bounds__(TX * TY)
     ez_stage_tiled(const FlatK<T> k, const FlatIO<T> io, const T As, const T Bs, const T ts,
                    const TileMap tm) {
-  constexpr int TX = 64, R = 2, LX = TX + 2 * R, LY = TY + 2 * R, NF = 7, NT = TX * TY;
+  constexpr int R = 2, LX = TX + 2 * R, LY = TY + 2 * R, NF = 7, NT = TX * TY;
   constexpr int H = 2 * R * (TX + TY);
   constexpr int HPT = (H + NT - 1) / NT;
   constexpr int Q = 2 * R + 1;
@@ -187,10 +187,10 @@ __global__ void __launch_bounds__(64 * TY)
   }
 }
 
-template <typename T, int TY, bool STAR>
-__global__ void __launch_bounds__(64 * TY)
+template <typename T, int TX, int TY, bool STAR>
+__global__ void __launch_bounds__(TX * TY)
     ez_ko_tiled(const FlatK<T> k, const FlatIO<T> io, const T tko, const TileMap tm) {
-  constexpr int TX = 64, R = 3, LX = TX + 2 * R, LY = TY + 2 * R, NF = 6, NT = TX * TY;
+  constexpr int R = 3, LX = TX + 2 * R, LY = TY + 2 * R, NF = 6, NT = TX * TY;
   constexpr int H = 2 * R * (TX + TY);
   constexpr int HPT = (H + NT - 1) / NT;
   constexpr int Q = 2 * R + 1;

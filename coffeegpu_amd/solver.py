@@ -271,7 +271,10 @@ class FieldSolverEZ:
                 "nonfinite": out[4]}
 
     def set_schedule(self, variant: int = 0, zchunk: int = 0):
-        """0 = LDS-tiled z-marching kernels (default), 1 = one thread per cell."""
+        """0 = LDS-tiled z-marching kernels (default tile), 1 = one thread per
+        cell, 2/3/4 = 64-wide tiles of 4/8/16 rows, 5/6 = 128/256-wide tiles,
+        7 = 128 x 4.
+        Every schedule gives bitwise the same results."""
         _lib.check(self._lib.coffee_set_schedule(self._ctx, int(variant), int(zchunk)), self._ctx)
 
     def set_timing(self, on: bool):

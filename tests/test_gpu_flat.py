@@ -237,7 +237,7 @@ def test_schedules_bitwise_identical(name, dtype):
     sp, U0 = CASES[name](dtype)
     bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
     outs = []
-    for variant, zchunk in ((1, 0), (0, 0), (0, 3), (0, 5)):
+    for variant, zchunk in ((1, 0), (0, 0), (0, 3), (0, 5), (2, 0), (4, 0), (6, 4), (7, 0)):
         env = SimEnvironment(sp, dtype, "cuda:0")
         data = SimData(env)
         data.set_state(U0)
