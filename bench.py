@@ -392,6 +392,8 @@ def main():
     bytes_per_launch = SCALARS_PER_CELL["ez_stage"] * esz * cells
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = load_traffic(f"ez_stage_{n}_{args.dtype}_traffic.json")
+    if traffic and traffic.get("variant", 0) != args.variant:
+        traffic = None  # measured for another schedule
     kernels = _kernel_table(timing, SCALARS_PER_CELL, esz, cells)
     step_gbs = SCALARS_PER_CELL_UPDATE * esz * value / 1e9
 
