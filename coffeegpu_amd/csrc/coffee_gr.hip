@@ -53,7 +53,6 @@ void fill_gr(const coffee_grid& g, const coffee_params& p, GrK<T>& k) {
   k.sz = g.dims[0] * g.dims[1];
   k.a = p.a;
   k.dt = (T)p.dt;
-  k.dt_d = p.dt;
   k.tau = (T)p.tau;
   k.KOfac = -(T)p.KOeps * (T)(-1.0 / 64.0);
   k.calc_current = p.calc_current;

@@ -42,7 +42,6 @@ struct GrK {
   double pmlscale;      // pmllen * delta[0] on every axis (boundary.cu:55-60)
   double sigpml;
   double r1, r2, dd;    // 0.7 rH, 0.01 rH, 0.2 rH (.cu:625-628)
-  double dt_d;
 };
 
 template <typename T>
@@ -179,49 +178,74 @@ __device__ __forceinline__ bool gr_in_outer(const GrK<T>& k, double x, double y,
                        z > k.xh[2] || z < k.xl[2]);
 }
 
-__device__ __forceinline__ double gr_pmlsigma(double x, double xl, double xh, double scale,
-                                              double sig0) {
-  // pmlsigma (boundary.cu:7-15)
+// pmlsigma (boundary.cu:7-15), in the field precision T as the reference's
+// Scalar builds evaluate it (positions and bounds are held in double)
+template <typename T>
+__device__ __forceinline__ T gr_pmlsigma(double x, double xl, double xh, T scale, T sig0) {
   if (x > xh) {
-    const double v = (x - xh) / scale;
+    const T v = (T)(x - xh) / scale;
     return sig0 * v * v * v;
   }
   if (x < xl) {
-    const double v = (xl - x) / scale;
+    const T v = (T)(xl - x) / scale;
     return sig0 * v * v * v;
   }
-  return 0.0;
+  return T(0);
+}
+
+// Damping factors of the layers: e^-sig and (1 - e^-sig) / sig.  In FP32 the
+// second comes from expm1, since 1 - e^-sig cancels for the small sig at the
+// inner edge of a layer; FP64 keeps the reference's expression.
+template <typename T>
+__device__ __forceinline__ void gr_damp(T sig, T& ex, T& w) {
+  if constexpr (sizeof(T) == 4) {
+    ex = expf(-sig);
+    w = -expm1f(-sig) / sig;
+  } else {
+    ex = exp(-sig);
+    w = (T(1) - ex) / sig;
+  }
+}
+
+// Cells whose step-start copy Un is saved in the first stage and read by
+// the absorbing layer in the last: the outer layer and the excision region.
+template <typename T>
+__device__ __forceinline__ bool gr_needs_un(const GrK<T>& k, double x, double y, double z,
+                                            double r) {
+  return gr_in_outer(k, x, y, z) || r < k.r1;
 }
 
 // kernel_boundary_absorbing_thread (boundary.cu:17-79) then
-// kernel_absorbing_inner (.cu:617-701) on D, B; P untouched.
+// kernel_absorbing_inner (.cu:617-701) on D, B; P untouched.  un: the cell's
+// Un (read only where gr_needs_un holds).
 template <typename T>
-__device__ __forceinline__ void gr_absorb(const GrK<T>& k, const GrIO<T>& io, int c, double x,
-                                          double y, double z, double r, T u[7]) {
-  const double tiny = (double)TinyOf<T>::v;
+__device__ __forceinline__ void gr_absorb(const GrK<T>& k, const T un_[7], double x, double y,
+                                          double z, double r, T u[7]) {
+  const T tiny = TinyOf<T>::v;
   if (gr_in_outer(k, x, y, z)) {
-    const double sig = gr_pmlsigma(x, k.xl[0], k.xh[0], k.pmlscale, k.sigpml) +
-                       gr_pmlsigma(y, k.xl[1], k.xh[1], k.pmlscale, k.sigpml) +
-                       gr_pmlsigma(z, k.xl[2], k.xh[2], k.pmlscale, k.sigpml);
+    const T scale = (T)k.pmlscale, sig0 = (T)k.sigpml;
+    const T sig = gr_pmlsigma<T>(x, k.xl[0], k.xh[0], scale, sig0) +
+                  gr_pmlsigma<T>(y, k.xl[1], k.xh[1], scale, sig0) +
+                  gr_pmlsigma<T>(z, k.xl[2], k.xh[2], scale, sig0);
     if (sig > tiny) {
-      const double e = exp(-sig);
-      const T ex = (T)e, w = (T)((1.0 - e) / sig);
+      T ex, w;
+      gr_damp(sig, ex, w);
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        const T un = io.Un[f][c];
+        const T un = un_[f];
         u[f] = ex * un + w * (u[f] - un);
       }
     }
   }
   if (r < k.r1) {
-    const double v = (k.r1 - r) / k.dd;
-    const double sig = k.sigpml * v * v * v * k.dt_d;
+    const T v = (T)(k.r1 - r) / (T)k.dd;
+    const T sig = (T)k.sigpml * v * v * v * k.dt;
     if (sig > tiny) {
-      const double e = exp(-sig);
-      const T ex = (T)e, w = (T)((1.0 - e) / sig);
+      T ex, w;
+      gr_damp(sig, ex, w);
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        const T un = io.Un[f][c];
+        const T un = un_[f];
         u[f] = ex * un + w * (u[f] - un);
       }
     }
@@ -273,12 +297,13 @@ constexpr int kGrDz[6] = {0, 1, 3, 4, 8, 11};
 // One LSRK stage at an interior cell from its derivatives (rk_step1
 // .cu:188-373, rk_step2 .cu:375-438, projections .cu:440-535, absorbing in
 // the last stage).  u: D, B, P at the cell (in), dprev: dU before the stage,
-// Ec: E_i at the cell.
+// Ec: E_i at the cell, un: the cell's Un (last stage, where gr_needs_un).
 template <typename T, bool FIRST, bool LAST, bool DIAG>
 __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io, int c, double x,
                                               double y, double z, const CksT<T>& m,
                                               const T ddx[7], const T ddy[7], const T ddz[7],
-                                              const T Ec[3], T u[7], const T dprev[7], T As, T Bs) {
+                                              const T Ec[3], T u[7], const T dprev[7],
+                                              const T un[7], T As, T Bs) {
   const T tiny = TinyOf<T>::v;
   // rot = curl / sqrt(g) = alpha * curl
   const T rotE[3] = {m.alpha * (ddy[1] - ddz[1]), m.alpha * (ddz[0] - ddx[1]),
@@ -322,14 +347,14 @@ __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io
     du[q] = As * du[q] + dt * (rotH[q] - J[q]);
   }
   du[6] = As * du[6] + dt * (-m.alpha * divB + dot3(beta, gP)) - m.alpha * u[6] / k.tau;
-  if (FIRST && (gr_in_outer(k, x, y, z) || m.r < k.r1)) {
+  if (FIRST && gr_needs_un(k, x, y, z, m.r)) {
 #pragma unroll
     for (int f = 0; f < 7; f++) io.Un[f][c] = u[f];
   }
 #pragma unroll
   for (int f = 0; f < 7; f++) u[f] = u[f] + Bs * du[f];
   gr_project(k, m, u);
-  if (LAST) gr_absorb(k, io, c, x, y, z, m.r, u);
+  if (LAST) gr_absorb(k, un, x, y, z, m.r, u);
 #pragma unroll
   for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
   if (!LAST) {
@@ -366,13 +391,16 @@ __device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io
   ddy[6] = gr_d1(P, c, sy, k.c12[1]);
   ddz[6] = gr_d1(P, c, sz, k.c12[2]);
   const T Ec[3] = {W[0][c], W[1][c], W[2][c]};
-  T u[7], dp[7];
+  T u[7], dp[7], un[7];
+  const bool need_un = LAST && gr_needs_un(k, x, y, z, m.r);
 #pragma unroll
   for (int f = 0; f < 7; f++) {
     u[f] = io.in[f][c];
     dp[f] = FIRST ? T(0) : io.dU[f][c];
+    un[f] = need_un ? io.Un[f][c] : T(0);
   }
-  gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, x, y, z, m, ddx, ddy, ddz, Ec, u, dp, As, Bs);
+  gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, x, y, z, m, ddx, ddy, ddz, Ec, u, dp, un, As,
+                                      Bs);
 }
 
 template <typename T, bool FIRST, bool LAST, bool DIAG>

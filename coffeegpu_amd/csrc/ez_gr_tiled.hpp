@@ -137,13 +137,22 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
         for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + ak * k.sz];
   }
-  T dp[7];
+  // dU (all but the first stage) and, in the last stage, Un of the current
+  // plane, prefetched one plane ahead
+  const double px = gr_pos(k, 0, ai), py = gr_pos(k, 1, aj);
+  T dp[7], up[7];
 #pragma unroll
-  for (int f = 0; f < 7; f++) dp[f] = T(0);
+  for (int f = 0; f < 7; f++) {
+    dp[f] = T(0);
+    up[f] = T(0);
+  }
   if (!FIRST && active) {
     const int c0 = col + (z0 + k.guard[2]) * k.sz;
 #pragma unroll
     for (int f = 0; f < 7; f++) dp[f] = io.dU[f][c0];
+    if (LAST && gr_needs_un(k, px, py, gr_pos(k, 2, z0 + k.guard[2]), mq[0].r))
+#pragma unroll
+      for (int f = 0; f < 7; f++) up[f] = io.Un[f][c0];
   }
 
   for (int z = z0; z < z1; z++) {
@@ -182,13 +191,19 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
           for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + off];
     }
-    T dn[7];
+    T dn[7], un[7];
 #pragma unroll
-    for (int f = 0; f < 7; f++) dn[f] = T(0);
+    for (int f = 0; f < 7; f++) {
+      dn[f] = T(0);
+      un[f] = T(0);
+    }
     if (!FIRST && active && z + 1 < z1) {
       const int c1 = col + (akz + 1) * k.sz;
 #pragma unroll
       for (int f = 0; f < 7; f++) dn[f] = io.dU[f][c1];
+      if (LAST && gr_needs_un(k, px, py, gr_pos(k, 2, akz + 1), mq[1].r))
+#pragma unroll
+        for (int f = 0; f < 7; f++) un[f] = io.Un[f][c1];
     }
     __syncthreads();
     if (active) {
@@ -205,9 +220,8 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
       for (int f = 0; f < 7; f++) u[f] = rawq[0][f];
       const int c = col + akz * k.sz;
-      gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, gr_pos(k, 0, ai), gr_pos(k, 1, aj),
-                                          gr_pos(k, 2, akz), mq[0], ddx, ddy, ddz, Ec, u, dp, As,
-                                          Bs);
+      gr_stage_core<T, FIRST, LAST, DIAG>(k, io, c, px, py, gr_pos(k, 2, akz), mq[0], ddx, ddy,
+                                          ddz, Ec, u, dp, up, As, Bs);
     }
     // advance the queues: plane z+3 enters
     {
@@ -225,6 +239,7 @@ __global__ void __launch_bounds__(64 * TY)
         rawq[2][f] = inc[f];
         inc[f] = nxt[f];
         dp[f] = dn[f];
+        up[f] = un[f];
       }
       mq[0] = mq[1];
       mq[1] = mq[2];
