@@ -784,14 +784,24 @@ __global__ void __launch_bounds__(256)
   if (run >= nruns) return;
   const SphRun<T>& R = runs[run];
   if (!R.active) return;
-  const long total = (long)R.dims0 * R.dims1;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % R.dims0), j = (int)(t / R.dims0);
-    const bool guard = i < R.g0 || i >= R.dims0 - R.g0 || j < R.g1 || j >= R.dims1 - R.g1;
-    if (!guard) continue;
+  // guard cells only: the g1 full rows below and above the interior, then
+  // the g0 cells left and right of every interior row
+  const int rows = 2 * R.g1 * R.dims0;
+  const int total = rows + 2 * R.g0 * R.n1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int i, j;
+    if (t < rows) {
+      const int r = t / R.dims0;
+      i = t % R.dims0;
+      j = r < R.g1 ? r : R.dims1 - 2 * R.g1 + r;
+    } else {
+      const int u = t - rows, c = u % (2 * R.g0);
+      j = R.g1 + u / (2 * R.g0);
+      i = c < R.g0 ? c : R.dims0 - 2 * R.g0 + c;
+    }
+    const unsigned o = (unsigned)(i + j * R.dims0);
 #pragma unroll
-    for (int q = 0; q < 7; q++) gst(R.U[1][q] + t, gld(R.U[0][q] + t));
+    for (int q = 0; q < 7; q++) gst_at(R.U[1][q], o, gld_at(R.U[0][q], o));
   }
 }
 
