@@ -1,7 +1,9 @@
 // U-Net building blocks for gfx950 (include/coffee_unet.h): BF16 activations
 // and weights, FP32 accumulation on the matrix cores.
 //
-// The 3x3 convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16: the
+// The 3x3 convolution is an implicit GEMM.  With Cin % 32 == 0 it runs on
+// v_mfma_f32_16x16x32_bf16 with the weights staged in LDS (conv3x3_k32
+// below); otherwise (the 16-channel input layer) on v_mfma_f32_16x16x16_bf16: the
 // M dimension is 16 pixels per wavefront (4 wavefronts = 64 pixels per
 // workgroup), N is 16*NB output channels, K runs over the 9 taps times the
 // input channels in steps of 16.  NHWC with channels padded to 16 makes every
@@ -113,6 +115,122 @@ __global__ void __launch_bounds__(256) conv3x3_mfma(const ConvArgs a) {
       if (a.out) a.out[p * a.Cout + co] = f2bf(val);
       if (a.out32) a.out32[p * a.Cout + co] = val;
     }
+  }
+}
+
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// The same convolution on v_mfma_f32_16x16x32_bf16 (twice the K of the
+// 16x16x16 instruction per issue) for Cin % 32 == 0.  A workgroup of 4
+// wavefronts computes 64*MB pixels x 16*NB output channels; K runs over the 9
+// taps x Cin in steps of 32.  The weight slice of a step (16*NB channels x 32
+// input channels) is staged in LDS, double-buffered and shared by the 4
+// wavefronts; each wavefront's input fragments (16 B per lane) and the next
+// weight slice are requested one step ahead.  Operand layout (wave64):
+//   A[i][k] from lane i + 16*(k/8), element k%8;
+//   B[k][j] from lane j + 16*(k/8), element k%8;
+//   D[i][j] in lane j + 16*(i/4), element i%4.
+template <int NB, int MB>
+__global__ void __launch_bounds__(256) conv3x3_k32(const ConvArgs a) {
+  constexpr int LDW = 40;  // LDS row: 32 input channels + 8 of padding (bank spread)
+  __shared__ __attribute__((aligned(16))) bf16 wl[2][16 * NB * LDW];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const long P = (long)a.N * a.H * a.W;
+  const long pixw = ((long)blockIdx.x * 4 + wave) * (16 * MB);
+  const int co0 = blockIdx.y * 16 * NB;
+  int pn[MB], py[MB], px[MB];
+  bool pv[MB];
+#pragma unroll
+  for (int m = 0; m < MB; m++) {
+    const long pa = pixw + m * 16 + li;
+    pv[m] = pa < P;
+    pn[m] = py[m] = px[m] = 0;
+    if (pv[m]) {
+      px[m] = (int)(pa % a.W);
+      const long t = pa / a.W;
+      py[m] = (int)(t % a.H);
+      pn[m] = (int)(t / a.H);
+    }
+  }
+  // weight staging: thread t loads 8 input channels (16 B) of row t/4
+  const int wrow = threadIdx.x >> 2, wpart = threadIdx.x & 3;
+  const bool wload = wrow < 16 * NB;
+  const bool wvalid = wload && co0 + wrow < a.Cout;
+  const int nci = a.Cin / 32, nk = 9 * nci;
+  auto load_w = [&](int s) {
+    s8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int tap = s / nci, c0 = (s - tap * nci) * 32;
+    if (wvalid) v = *(const s8*)(a.w + ((long)tap * a.Cout + co0 + wrow) * a.Cin + c0 + 8 * wpart);
+    return v;
+  };
+  auto load_a = [&](int s, s8 (&av)[MB]) {
+    const int tap = s / nci, c0 = (s - tap * nci) * 32;
+    int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    if (a.flip) {
+      dy = -dy;
+      dx = -dx;
+    }
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      const int yy = py[m] + dy, xx = px[m] + dx;
+      av[m] = s8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (pv[m] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+        av[m] = *(const s8*)(a.in + ((long)(pn[m] * a.H + yy) * a.W + xx) * a.Cin + c0 + 8 * lk);
+    }
+  };
+  f4 acc[MB][NB];
+#pragma unroll
+  for (int m = 0; m < MB; m++)
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[m][b] = f4{0.f, 0.f, 0.f, 0.f};
+  s8 acur[MB], anext[MB];
+  s8 wnext = load_w(0);
+  load_a(0, acur);
+  if (wload) *(s8*)&wl[0][wrow * LDW + 8 * wpart] = wnext;
+  __syncthreads();
+  for (int s = 0; s < nk; s++) {
+    if (s + 1 < nk) {
+      wnext = load_w(s + 1);
+      load_a(s + 1, anext);
+    }
+    const bf16* wb = wl[s & 1];
+    bf8 bfr[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      bfr[b] = __builtin_bit_cast(bf8, *(const s8*)&wb[(b * 16 + li) * LDW + 8 * lk]);
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      const bf8 af = __builtin_bit_cast(bf8, acur[m]);
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+        acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[b], acc[m][b], 0, 0, 0);
+    }
+    if (s + 1 < nk) {
+      // buffer (s + 1) & 1 was last read in step s - 1, before its barrier
+      if (wload) *(s8*)&wl[(s + 1) & 1][wrow * LDW + 8 * wpart] = wnext;
+#pragma unroll
+      for (int m = 0; m < MB; m++) acur[m] = anext[m];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int co = co0 + b * 16 + li;
+    if (co >= a.Cout) continue;
+    const float bb = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const long p = pixw + m * 16 + 4 * lk + v;
+        if (p >= P) continue;
+        float val = acc[m][b][v] + bb;
+        if (a.relu) val = val > 0.f ? val : 0.f;
+        if (a.out) a.out[p * a.Cout + co] = f2bf(val);
+        if (a.out32) a.out32[p * a.Cout + co] = val;
+      }
   }
 }
 
@@ -411,7 +529,25 @@ int coffee_unet_conv3x3(const void* in, const void* w, const float* bias, void* 
   const long P = (long)n * h * wd;
   const long bx = (P + 63) / 64;
   if (bx > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3: too many pixels");
-  if (cout % 64 == 0) {
+  const bool aligned16 = ((uintptr_t)in % 16 == 0) && ((uintptr_t)w % 16 == 0);
+  if (cin % 32 == 0 && aligned16) {
+    // K = 32 kernel; two 16-pixel row blocks per wavefront when that still
+    // leaves two workgroups per CU
+    const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    const long b2 = (P + 127) / 128, cb = cout / (16 * nb);
+    const bool mb2 = b2 * cb >= 512;
+    const unsigned gx = (unsigned)(mb2 ? b2 : bx);
+    const dim3 g(gx, (unsigned)cb);
+#define COFFEE_CONV32(NBV)                                              \
+  do {                                                                  \
+    if (mb2) conv3x3_k32<NBV, 2><<<g, 256, 0, s>>>(a);                 \
+    else conv3x3_k32<NBV, 1><<<g, 256, 0, s>>>(a);                     \
+  } while (0)
+    if (nb == 4) COFFEE_CONV32(4);
+    else if (nb == 2) COFFEE_CONV32(2);
+    else COFFEE_CONV32(1);
+#undef COFFEE_CONV32
+  } else if (cout % 64 == 0) {
     conv3x3_mfma<4><<<dim3((unsigned)bx, cout / 64), 256, 0, s>>>(a);
   } else if (cout % 32 == 0) {
     conv3x3_mfma<2><<<dim3((unsigned)bx, cout / 32), 256, 0, s>>>(a);

@@ -55,10 +55,15 @@ def test_bf16_conversion_round_to_nearest_even():
     assert np.array_equal(out.cpu().numpy().view(np.uint16), bf16_bits(x))
 
 
-@pytest.mark.parametrize("cin,cout,flip", [(16, 16, 0), (16, 48, 0), (32, 64, 0), (64, 32, 1)])
-def test_conv3x3_matches_oracle(cin, cout, flip):
-    rng = np.random.default_rng(cin + cout + flip)
-    n, h, w = 2, 7, 9
+# cin 16: the 16x16x16 kernel; cin % 32 == 0: the K = 32 kernel with LDS-staged
+# weights (one or two 16-pixel blocks per wavefront, 1/2/4 channel blocks,
+# partial pixel and channel tiles)
+@pytest.mark.parametrize("cin,cout,flip,n,h,w", [
+    (16, 16, 0, 2, 7, 9), (16, 48, 0, 2, 7, 9), (32, 64, 0, 2, 7, 9), (64, 32, 1, 2, 7, 9),
+    (96, 80, 1, 2, 7, 9), (32, 32, 0, 2, 7, 9), (32, 64, 0, 1, 256, 256),
+    (64, 16, 1, 1, 257, 259)])
+def test_conv3x3_matches_oracle(cin, cout, flip, n, h, w):
+    rng = np.random.default_rng(cin + cout + flip + h)
     x = bf16_round(rng.standard_normal((n, h, w, cin)))
     W = bf16_round(rng.standard_normal((3, 3, cin, cout)) * 0.2)
     b = rng.standard_normal(cout).astype(np.float32).astype(np.float64)
