@@ -277,6 +277,91 @@ __global__ void __launch_bounds__(64)
   for (int v = 0; v < 4; v++) dst[(long)(cob * 16 + 4 * lk + v) * Cin + cib * 16 + li] = acc[v];
 }
 
+// Weight gradient on v_mfma_f32_16x16x32_bf16: for one tap and one pixel
+// range a workgroup (4 wavefronts) computes a 64 (co) x 64 (ci) tile of
+// dW[tap][co][ci] = sum_p dz[p][co] x[p + tap][ci] (GEMM with K = pixels).
+// Each step stages 32 pixels of dz and of the shifted input in LDS, transposed
+// to channel-major rows so that a lane's 8 consecutive pixels (one MFMA
+// fragment) are one 16-byte read; the global loads (16 B = 8 channels of one
+// pixel per thread) for the next step are issued before the current step's
+// MFMAs.  Wavefront w computes channels co0 + 16 w .. + 15 against the 64
+// input channels (4 MFMAs per step).  Channels beyond Cout / Cin and pixels
+// beyond the range or the image are zero.
+__global__ void __launch_bounds__(256)
+    wgrad_k32(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
+              int N, int H, int W, int Cin, int Cout, long chunk) {
+  constexpr int LDP = 40;  // LDS row: 32 pixels + 8 of padding (bank spread)
+  __shared__ __attribute__((aligned(16))) bf16 dzt[2][64 * LDP];
+  __shared__ __attribute__((aligned(16))) bf16 xt[2][64 * LDP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int split = blockIdx.x, tap = blockIdx.y;
+  const int ncib = (Cin + 63) / 64;
+  const int co0 = (blockIdx.z / ncib) * 64, ci0 = (blockIdx.z % ncib) * 64;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const long P = (long)N * H * W;
+  const long p_begin = (long)split * chunk;
+  const long p_end = p_begin + chunk < P ? p_begin + chunk : P;
+  // staging: thread t moves 8 channels (16 B) of pixel t % 32 of the step,
+  // channels 8 (t / 32) ..; consecutive lanes write consecutive LDS columns
+  const int sp = threadIdx.x & 31, sc = (threadIdx.x >> 5) * 8;
+  const bool dz_ok = co0 + sc < Cout, x_ok = ci0 + sc < Cin;
+  auto fetch = [&](long k0, s8& vdz, s8& vx) {
+    vdz = s8{0, 0, 0, 0, 0, 0, 0, 0};
+    vx = s8{0, 0, 0, 0, 0, 0, 0, 0};
+    const long p = k0 + sp;
+    if (p >= p_end) return;
+    if (dz_ok) vdz = *(const s8*)(dz + p * Cout + co0 + sc);
+    const int px = (int)(p % W);
+    const long t = p / W;
+    const int py = (int)(t % H), pn = (int)(t / H);
+    const int yy = py + dy, xx = px + dx;
+    if (x_ok && yy >= 0 && yy < H && xx >= 0 && xx < W)
+      vx = *(const s8*)(x + ((long)(pn * H + yy) * W + xx) * Cin + ci0 + sc);
+  };
+  auto stage = [&](int buf, const s8& vdz, const s8& vx) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      dzt[buf][(sc + e) * LDP + sp] = (bf16)vdz[e];
+      xt[buf][(sc + e) * LDP + sp] = (bf16)vx[e];
+    }
+  };
+  f4 acc[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  s8 ndz, nx;
+  fetch(p_begin, ndz, nx);
+  stage(0, ndz, nx);
+  __syncthreads();
+  int buf = 0;
+  for (long k0 = p_begin; k0 < p_end; k0 += 32) {
+    const bool more = k0 + 32 < p_end;
+    if (more) fetch(k0 + 32, ndz, nx);
+    const bf8 af = __builtin_bit_cast(bf8, *(const s8*)&dzt[buf][(wave * 16 + li) * LDP + 8 * lk]);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const bf8 bfr = __builtin_bit_cast(bf8, *(const s8*)&xt[buf][(b * 16 + li) * LDP + 8 * lk]);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc[b], 0, 0, 0);
+    }
+    // buffer buf ^ 1 was last read in the previous step, before its barrier
+    if (more) stage(buf ^ 1, ndz, nx);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // D[i = co][j = ci]: lane holds co = co0 + 16 wave + 4 lk + v, ci = ci0 + 16 b + li
+  float* dst = part + ((long)split * 9 + tap) * Cout * Cin;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int ci = ci0 + b * 16 + li;
+    if (ci >= Cin) continue;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int co = co0 + wave * 16 + 4 * lk + v;
+      if (co < Cout) dst[(long)co * Cin + ci] = acc[b][v];
+    }
+  }
+}
+
 __global__ void wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, long n,
                              int nsplit) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -566,8 +651,12 @@ int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int
   long chunk = (P + nsplit - 1) / nsplit;
   chunk = (chunk + 15) / 16 * 16;
   const int used = (int)((P + chunk - 1) / chunk);
-  wgrad_mfma<<<dim3(used, 9, (cout / 16) * (cin / 16)), 64, 0, s>>>(
-      (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
+  if ((uintptr_t)x % 16 == 0 && (uintptr_t)dz % 16 == 0)
+    wgrad_k32<<<dim3(used, 9, ((cout + 63) / 64) * ((cin + 63) / 64)), 256, 0, s>>>(
+        (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
+  else
+    wgrad_mfma<<<dim3(used, 9, (cout / 16) * (cin / 16)), 64, 0, s>>>(
+        (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
   int rc = launch_check("conv3x3_wgrad");
   if (rc) return rc;
   const long nw = 9L * cout * cin;
