@@ -728,49 +728,42 @@ __global__ void __launch_bounds__(256)
     const int i = ii + R.g0;
     const int s = R.dims0;
     T* const* U = R.U[buf];
-    // U[f][dst] = sign * U[f][src]
-    auto mv = [&](int f, int dst, int src, T sign) { gst(U[f] + dst, sign * gld(U[f] + src)); };
-    auto zero = [&](int f, int dst) { gst(U[f] + dst, T(0)); };
+    // One axis row c: E_r, B_r, P from the neighbouring row c + dir s, the
+    // other components zero; guard rows c - dir l s (l = 1..3) mirrored from
+    // c + dir l s with parity + for r components, - for theta and phi.  The
+    // source rows are all read before any row is written (they are disjoint
+    // from the written ones), so the copies cost one memory round trip.
+    auto axis = [&](int c, int dir) {
+      T v[3][6];
+#pragma unroll
+      for (int l = 0; l < 3; l++)
+#pragma unroll
+        for (int f = 0; f < 6; f++) v[l][f] = gld(U[f] + c + dir * (l + 1) * s);
+      const T p1 = gld(U[6] + c + dir * s);
+      gst(U[0] + c, v[0][0]);
+      gst(U[1] + c, T(0));
+      gst(U[2] + c, T(0));
+      gst(U[3] + c, v[0][3]);
+      gst(U[4] + c, T(0));
+      gst(U[5] + c, T(0));
+      gst(U[6] + c, p1);
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        const int d = c - dir * (l + 1) * s;
+        gst(U[0] + d, v[l][0]);
+        gst(U[1] + d, -v[l][1]);
+        gst(U[2] + d, -v[l][2]);
+        gst(U[3] + d, v[l][3]);
+        gst(U[4] + d, -v[l][4]);
+        gst(U[5] + d, -v[l][5]);
+      }
+    };
     int j = R.g1;
-    int c = i + j * s;
     T y = R.y0 + R.dy * T(j - R.g1);
-    if (fabs(y) < R.dy / T(2)) {
-      mv(0, c, c + s, T(1));
-      zero(1, c);
-      zero(2, c);
-      mv(3, c, c + s, T(1));
-      zero(4, c);
-      zero(5, c);
-      mv(6, c, c + s, T(1));
-      for (int l = 1; l <= 3; ++l) {
-        mv(0, c - l * s, c + l * s, T(1));
-        mv(1, c - l * s, c + l * s, T(-1));
-        mv(2, c - l * s, c + l * s, T(-1));
-        mv(3, c - l * s, c + l * s, T(1));
-        mv(4, c - l * s, c + l * s, T(-1));
-        mv(5, c - l * s, c + l * s, T(-1));
-      }
-    }
+    if (fabs(y) < R.dy / T(2)) axis(i + j * s, 1);
     j = R.dims1 - R.g1;
-    c = i + j * s;
     y = R.y0 + R.dy * T(j - R.g1);
-    if (fabs(y - T(M_PI)) < R.dy / T(2)) {
-      mv(0, c, c - s, T(1));
-      zero(1, c);
-      zero(2, c);
-      mv(3, c, c - s, T(1));
-      zero(4, c);
-      zero(5, c);
-      mv(6, c, c - s, T(1));
-      for (int l = 1; l <= 3; ++l) {
-        mv(0, c + l * s, c - l * s, T(1));
-        mv(1, c + l * s, c - l * s, T(-1));
-        mv(2, c + l * s, c - l * s, T(-1));
-        mv(3, c + l * s, c - l * s, T(1));
-        mv(4, c + l * s, c - l * s, T(-1));
-        mv(5, c + l * s, c - l * s, T(-1));
-      }
-    }
+    if (fabs(y - T(M_PI)) < R.dy / T(2)) axis(i + j * s, -1);
   }
 }
 

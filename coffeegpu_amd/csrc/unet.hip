@@ -371,19 +371,48 @@ __global__ void wgrad_reduce(const float* __restrict__ part, float* __restrict__
   }
 }
 
-__global__ void __launch_bounds__(256) bias_grad(const bf16* __restrict__ dz, float* __restrict__ db,
-                                                 long P, int C) {
+// Bias gradient db[c] = sum_p dz[p][c] in two deterministic passes.  Pass 1:
+// block b sums the pixels [b chunk, (b + 1) chunk) of every channel, threads
+// laid out channel-fastest so that a pixel's channels are one coalesced row
+// (channel groups of up to 256; 256 / group pixel rows in parallel, summed in
+// a fixed order).  Pass 2 adds the block partials in block order.
+__global__ void __launch_bounds__(256) bias_grad_part(const bf16* __restrict__ dz,
+                                                      float* __restrict__ part, long P, int C,
+                                                      long chunk) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
-  float s = 0.f;
-  for (long p = threadIdx.x; p < P; p += blockDim.x) s += bf2f(dz[p * C + c]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int m = 128; m > 0; m >>= 1) {
-    if ((int)threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+  const long p0 = (long)blockIdx.x * chunk;
+  const long p1 = p0 + chunk < P ? p0 + chunk : P;
+  for (int cb = 0; cb < C; cb += 256) {
+    const int cw = C - cb < 256 ? C - cb : 256;
+    const int rows = 256 / cw;
+    const int c = (int)threadIdx.x % cw, r = (int)threadIdx.x / cw;
+    float s = 0.f;
+    if (r < rows)
+      for (long p = p0 + r; p < p1; p += rows) s += bf2f(dz[p * C + cb + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < cw) {
+      float t = 0.f;
+      for (int k = 0; k < rows; k++) t += red[k * cw + threadIdx.x];
+      part[(long)blockIdx.x * C + cb + threadIdx.x] = t;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[c] = red[0];
+}
+
+// Pass 2: 4 partial rows x 64 channels per block, each thread adding every
+// 4th block partial, then the 4 row sums in row order.
+__global__ void __launch_bounds__(256) bias_grad_sum(const float* __restrict__ part,
+                                                     float* __restrict__ db, int nb, int C) {
+  __shared__ float red[256];
+  const int cl = (int)threadIdx.x & 63, r = (int)threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  float s = 0.f;
+  if (c < C)
+    for (int b = r; b < nb; b += 4) s += part[(long)b * C + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (r == 0 && c < C) db[c] = ((red[cl] + red[64 + cl]) + red[128 + cl]) + red[192 + cl];
 }
 
 __global__ void combine_kernel(bf16* __restrict__ out, int ost, int ooff, const bf16* __restrict__ a,
@@ -663,8 +692,22 @@ int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int
   wgrad_reduce<<<grid_for(nw), 256, 0, s>>>(partial, dW, nw, used);
   rc = launch_check("wgrad_reduce");
   if (rc || !db) return rc;
-  bias_grad<<<cout, 256, 0, s>>>((const bf16*)dz, db, P, cout);
-  return launch_check("bias_grad");
+  // the weight partials have been reduced: their buffer holds the bias
+  // partials, one row of cout per block (at most nsplit x 9 x cin rows); a
+  // block covers 64 pixels per thread row
+  const long rows = 256 / (cout < 256 ? cout : 256);
+  const long cap = (long)nsplit * 9 * cin;
+  long bchunk = rows * 64;
+  long nbb = (P + bchunk - 1) / bchunk;
+  if (nbb > cap) {
+    bchunk = (P + cap - 1) / cap;
+    nbb = (P + bchunk - 1) / bchunk;
+  }
+  bias_grad_part<<<(unsigned)nbb, 256, 0, s>>>((const bf16*)dz, partial, P, cout, bchunk);
+  rc = launch_check("bias_grad_part");
+  if (rc) return rc;
+  bias_grad_sum<<<(cout + 63) / 64, 256, 0, s>>>(partial, db, (int)nbb, cout);
+  return launch_check("bias_grad_sum");
 }
 
 int coffee_unet_combine(void* out, int ost, int ooff, const void* a, int ast, int aoff,

@@ -80,13 +80,13 @@ def test_conv3x3_matches_oracle(cin, cout, flip, n, h, w):
     assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
 
 
-def test_conv3x3_wgrad_matches_oracle():
-    rng = np.random.default_rng(11)
-    n, h, w, cin, cout = 2, 6, 10, 32, 16
+@pytest.mark.parametrize("n,h,w,cin,cout,nsplit", [(2, 6, 10, 32, 16, 3), (1, 40, 52, 64, 48, 7),
+                                                    (2, 12, 16, 16, 272, 2)])
+def test_conv3x3_wgrad_matches_oracle(n, h, w, cin, cout, nsplit):
+    rng = np.random.default_rng(11 + cout)
     x = bf16_round(rng.standard_normal((n, h, w, cin)))
     dz = bf16_round(rng.standard_normal((n, h, w, cout)))
     _, dW, db = UO.conv3x3_bwd(x, np.zeros((3, 3, cin, cout)), dz)
-    nsplit = 3
     part = torch.empty(nsplit * 9 * cout * cin, dtype=torch.float32, device="cuda:0")
     gw = torch.empty(9 * cout * cin, dtype=torch.float32, device="cuda:0")
     gb = torch.empty(cout, dtype=torch.float32, device="cuda:0")
