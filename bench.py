@@ -55,9 +55,11 @@ def parse():
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0 = auto)")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=10)
-    ap.add_argument("--workload", choices=["alfven3d", "sph-sweep", "gr-wald"], default="alfven3d",
+    ap.add_argument("--workload", choices=["alfven3d", "sph-sweep", "gr-wald", "unet-conv"],
+                    default="alfven3d",
                     help="alfven3d: the headline flat 512^3 run (default); sph-sweep: batched 2D "
-                         "pulsar runs; gr-wald: Kerr-Schild Wald problem")
+                         "pulsar runs; gr-wald: Kerr-Schild Wald problem; unet-conv: the U-Net's "
+                         "BF16 MFMA convolution and weight-gradient kernels")
     ap.add_argument("--runs", type=int, default=256, help="sph-sweep: number of runs")
     ap.add_argument("--tile", type=int, default=-1,
                     help="sph-sweep: tile index (default: autotuned)")
@@ -254,6 +256,68 @@ GR_SCALARS = {"gr_stage": 28, "gr_stage_first": 21, "gr_stage_last": 21, "gr_ko"
               "gr_point": 18}
 
 
+UNET_SHAPES = [(8, 128, 64, 32, 32), (8, 256, 128, 64, 64), (8, 128, 64, 128, 128),
+               (8, 64, 32, 256, 256), (8, 16, 8, 256, 256)]
+
+
+def run_unet_conv(args):
+    """Forward 3x3 convolution and weight + bias gradient of the surrogate's
+    kernels at U-Net layer shapes (N images x H x W, Cin -> Cout), HIP events
+    around `--steps` launches each; FLOP = 2 N H W Cin Cout 9 per launch."""
+    import ctypes
+
+    import torch
+
+    from coffeegpu_amd import _lib
+
+    L = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    reps = max(1, args.steps)
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            _lib.check(fn())
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    rows = []
+    for (n, h, w, cin, cout) in UNET_SHAPES:
+        x = torch.randn(n * h * w * cin, device="cuda").to(torch.bfloat16)
+        wt = torch.randn(9 * cout * cin, device="cuda").to(torch.bfloat16)
+        b = torch.zeros(cout, device="cuda")
+        out = torch.empty(n * h * w * cout, device="cuda", dtype=torch.bfloat16)
+        dz = torch.randn(n * h * w * cout, device="cuda").to(torch.bfloat16)
+        nsplit = max(1, min(256, n * h * w // 512))
+        part = torch.empty(nsplit * 9 * cout * cin, device="cuda")
+        gw = torch.empty(9 * cout * cin, device="cuda")
+        gb = torch.empty(cout, device="cuda")
+        ms_f = timed(lambda: L.coffee_unet_conv3x3(p(x), p(wt), p(b), p(out), None, n, h, w, cin,
+                                                   cout, 0, 1, s))
+        ms_w = timed(lambda: L.coffee_unet_conv3x3_wgrad(p(x), p(dz), p(part), nsplit, p(gw), p(gb),
+                                                         n, h, w, cin, cout, s))
+        fl = 2.0 * n * h * w * cin * cout * 9
+        rows.append({"shape": [n, h, w, cin, cout], "conv_us": ms_f * 1e3,
+                     "conv_TFLOPs": fl / ms_f / 1e9, "wgrad_us": ms_w * 1e3,
+                     "wgrad_TFLOPs": fl / ms_w / 1e9})
+    peak = 2500.0
+    best = max(r["conv_TFLOPs"] for r in rows)
+    out = {"metric": "TFLOP/s (U-Net BF16 3x3 convolution, forward, 1 GPU)", "value": best,
+           "unit": "TFLOP/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "higher_is_better": True, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "unet-conv layer shapes N x H x W, Cin -> Cout"},
+           "roofline": {"bound": "mfma", "achieved": best, "peak": peak, "unit": "TFLOP/s",
+                        "frac": best / peak, "traffic": None, "kernel": "conv3x3"},
+           "shapes": rows}
+    print(json.dumps(out))
+
+
 def run_gr(args):
     import torch
 
@@ -338,6 +402,8 @@ def main():
         sys.exit(2)
     if args.workload == "sph-sweep":
         return run_sph_sweep(args)
+    if args.workload == "unet-conv":
+        return run_unet_conv(args)
     if args.workload == "gr-wald":
         return run_gr(args)
     import torch
