@@ -362,13 +362,24 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Sum of m values v(k), k = 0 .. m-1, as 8 interleaved partial sums (k mod 8)
+// combined pairwise: a fixed order (deterministic results) whose loads do
+// not wait on each other.
+template <typename F>
+__device__ __forceinline__ float sum8(int m, F&& v) {
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int k = 0;
+  for (; k + 8 <= m; k += 8)
+#pragma unroll
+    for (int u = 0; u < 8; u++) s[u] += v(k + u);
+  for (int u = 0; k + u < m; u++) s[u] += v(k + u);
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 __global__ void wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, long n,
                              int nsplit) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int k = 0; k < nsplit; k++) s += part[(long)k * n + i];
-    dW[i] = s;
-  }
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dW[i] = sum8(nsplit, [&](int k) { return part[(long)k * n + i]; });
 }
 
 // Bias gradient db[c] = sum_p dz[p][c] in two deterministic passes.  Pass 1:
@@ -401,15 +412,17 @@ __global__ void __launch_bounds__(256) bias_grad_part(const bf16* __restrict__ d
 }
 
 // Pass 2: 4 partial rows x 64 channels per block, each thread adding every
-// 4th block partial, then the 4 row sums in row order.
+// 4th block partial (sum8), then the 4 row sums in row order.
 __global__ void __launch_bounds__(256) bias_grad_sum(const float* __restrict__ part,
                                                      float* __restrict__ db, int nb, int C) {
   __shared__ float red[256];
   const int cl = (int)threadIdx.x & 63, r = (int)threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + cl;
   float s = 0.f;
-  if (c < C)
-    for (int b = r; b < nb; b += 4) s += part[(long)b * C + c];
+  if (c < C) {
+    const int m = nb > r ? (nb - r + 3) / 4 : 0;  // partials r, r + 4, ...
+    s = sum8(m, [&](int k) { return part[(long)(r + 4 * k) * C + c]; });
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   if (r == 0 && c < C) db[c] = ((red[cl] + red[64 + cl]) + red[128 + cl]) + red[192 + cl];
