@@ -41,7 +41,8 @@ struct TileTag {
 #define COFFEE_SPH_TILES(X)                                                                      \
   X(0, 64, 4, 0) X(1, 64, 8, 0) X(2, 128, 2, 0) X(3, 32, 8, 0) X(4, 256, 1, 0) X(5, 64, 8, 1)    \
   X(6, 64, 16, 1) X(7, 128, 4, 1) X(8, 32, 16, 1) X(9, 64, 4, 1) X(10, 128, 64, 2)              \
-  X(11, 256, 32, 2) X(12, 256, 64, 2) X(13, 256, 128, 2) X(14, 512, 64, 2) X(15, 128, 128, 2)
+  X(11, 256, 32, 2) X(12, 256, 64, 2) X(13, 256, 128, 2) X(14, 512, 64, 2) X(15, 128, 128, 2) \
+  X(16, 512, 128, 2) X(17, 512, 256, 2)
 #define COFFEE_SPH_TILE_ENTRY(n, tx, ty, k) {tx, ty, k},
 const Tile kTiles[] = {COFFEE_SPH_TILES(COFFEE_SPH_TILE_ENTRY)};
 #undef COFFEE_SPH_TILE_ENTRY
