@@ -49,6 +49,13 @@ __device__ __forceinline__ int amap(int v, int n, int g, int per) {
   return v + g;
 }
 
+// Whether a tiled kernel keeps the x-y halo of R+1 planes in registers
+// (fetched with the centre columns) or of one plane.
+template <typename T, bool STAR, int HPT, int NT>
+constexpr bool deep_halo() {
+  return !STAR && HPT == 1 && sizeof(T) * NT <= 4096;
+}
+
 // Halo cell h (0 <= h < 2R(TX+TY)) of a TX x TY tile: R rows below/above
 // the tile (x-extent TX), then R columns left/right (y-extent TY).
 template <int TX, int TY, int R>
@@ -113,14 +120,23 @@ __global__ void __launch_bounds__(TX * TY)
 #pragma unroll
     for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
   }
-  T hv[HPT][NF];
-  {
-    const int off = amap(z0, k.n[2], k.guard[2], k.per[2]) * k.sz;
+  // hv[d] = halo of plane z+d.  The halo of a plane is fetched HD = R+1
+  // planes ahead, in the same iteration as that plane's centre columns, i.e.
+  // when the neighbouring tiles fetch the same cells as their own centre, so
+  // the shared lines are still in this XCD's L2 (fetched a plane later they
+  // have been evicted by the 28 plane-streams every tile moves per plane).
+  // Only where the registers are free: the STAR kernels, tiles with more
+  // than one halo cell per thread and 1024-thread FP64 tiles keep one plane.
+  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + 1 : 1;
+  T hv[HD][HPT][NF];
+#pragma unroll
+  for (int d = 0; d < HD; d++) {
+    const int off = amap(z0 + d, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
     for (int h = 0; h < HPT; h++)
       if (hok[h])
 #pragma unroll
-        for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+        for (int f = 0; f < NF; f++) hv[d][h][f] = io.in[f][hcol[h] + off];
   }
   // dU of the current plane, prefetched one plane ahead
   T dp[7];
@@ -145,14 +161,20 @@ __global__ void __launch_bounds__(TX * TY)
     for (int h = 0; h < HPT; h++)
       if (hok[h])
 #pragma unroll
-        for (int f = 0; f < NF; f++) lds[buf][f][hslot[h]] = hv[h][f];
-    if (z + 1 < z1) {
-      const int off = amap(z + 1, k.n[2], k.guard[2], k.per[2]) * k.sz;
+        for (int f = 0; f < NF; f++) lds[buf][f][hslot[h]] = hv[0][h][f];
+#pragma unroll
+    for (int d = 0; d + 1 < HD; d++)
+#pragma unroll
+      for (int h = 0; h < HPT; h++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) hv[d][h][f] = hv[d + 1][h][f];
+    if (z + HD < z1) {
+      const int off = amap(z + HD, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
       for (int h = 0; h < HPT; h++)
         if (hok[h])
 #pragma unroll
-          for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+          for (int f = 0; f < NF; f++) hv[HD - 1][h][f] = io.in[f][hcol[h] + off];
     }
     T dn[7];
 #pragma unroll
@@ -232,14 +254,18 @@ __global__ void __launch_bounds__(TX * TY)
 #pragma unroll
     for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
   }
-  T hv[HPT][NF];
-  {
-    const int off = amap(z0, k.n[2], k.guard[2], k.per[2]) * k.sz;
+  // halo of planes z .. z+HD-1, fetched with the centre columns (see
+  // ez_stage_tiled)
+  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + 1 : 1;
+  T hv[HD][HPT][NF];
+#pragma unroll
+  for (int d = 0; d < HD; d++) {
+    const int off = amap(z0 + d, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
     for (int h = 0; h < HPT; h++)
       if (hok[h])
 #pragma unroll
-        for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+        for (int f = 0; f < NF; f++) hv[d][h][f] = io.in[f][hcol[h] + off];
   }
   T pc = T(0);
   if (active) pc = io.in[6][col + (z0 + k.guard[2]) * k.sz];
@@ -259,14 +285,20 @@ __global__ void __launch_bounds__(TX * TY)
     for (int h = 0; h < HPT; h++)
       if (hok[h])
 #pragma unroll
-        for (int f = 0; f < NF; f++) lds[buf][f][hslot[h]] = hv[h][f];
-    if (z + 1 < z1) {
-      const int off = amap(z + 1, k.n[2], k.guard[2], k.per[2]) * k.sz;
+        for (int f = 0; f < NF; f++) lds[buf][f][hslot[h]] = hv[0][h][f];
+#pragma unroll
+    for (int d = 0; d + 1 < HD; d++)
+#pragma unroll
+      for (int h = 0; h < HPT; h++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) hv[d][h][f] = hv[d + 1][h][f];
+    if (z + HD < z1) {
+      const int off = amap(z + HD, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
       for (int h = 0; h < HPT; h++)
         if (hok[h])
 #pragma unroll
-          for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
+          for (int f = 0; f < NF; f++) hv[HD - 1][h][f] = io.in[f][hcol[h] + off];
     }
     T pn = T(0);
     if (active && z + 1 < z1) pn = io.in[6][c + k.sz];
