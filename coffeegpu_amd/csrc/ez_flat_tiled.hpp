@@ -49,11 +49,19 @@ __device__ __forceinline__ int amap(int v, int n, int g, int per) {
   return v + g;
 }
 
-// Whether a tiled kernel keeps the x-y halo of R+1 planes in registers
+// Whether a tiled kernel keeps the x-y halo of R+LA planes in registers
 // (fetched with the centre columns) or of one plane.
 template <typename T, bool STAR, int HPT, int NT>
 constexpr bool deep_halo() {
   return !STAR && HPT == 1 && sizeof(T) * NT <= 4096;
+}
+
+// Planes of look-ahead of the centre columns (LA) of the stage kernels: two
+// where the registers allow (FP32 kernels with the deep halo, but not the
+// last stage, whose absorbing-layer operands take the room), else one.
+template <typename T, bool STAR, bool LAST, int HPT, int NT>
+constexpr int centre_lookahead() {
+  return deep_halo<T, STAR, HPT, NT>() && sizeof(T) == 4 && !LAST ? 2 : 1;
 }
 
 // Halo cell h (0 <= h < 2R(TX+TY)) of a TX x TY tile: R rows below/above
@@ -111,23 +119,25 @@ __global__ void __launch_bounds__(TX * TY)
     hslot[h] = (hy + R) * LX + (hx + R);
   }
 
-  // q[0..Q-1] = planes z-R..z+R; q[Q] receives plane z+R+1 during the
-  // iteration of plane z, so every global load has a full iteration to land.
-  T q[Q + 1][NF];
+  // q[0..Q-1] = planes z-R..z+R; q[Q..Q+LA-1] hold the next LA planes, the
+  // last of which is requested during the iteration of plane z, so every
+  // global load has LA full iterations to land.
+  constexpr int LA = centre_lookahead<T, STAR, LAST, HPT, NT>();
+  T q[Q + LA][NF];
 #pragma unroll
-  for (int d = 0; d < Q; d++) {
+  for (int d = 0; d < Q + LA - 1; d++) {
     const int off = amap(z0 - R + d, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
     for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
   }
-  // hv[d] = halo of plane z+d.  The halo of a plane is fetched HD = R+1
+  // hv[d] = halo of plane z+d.  The halo of a plane is fetched HD = R+LA
   // planes ahead, in the same iteration as that plane's centre columns, i.e.
   // when the neighbouring tiles fetch the same cells as their own centre, so
   // the shared lines are still in this XCD's L2 (fetched a plane later they
   // have been evicted by the 28 plane-streams every tile moves per plane).
   // Only where the registers are free: the STAR kernels, tiles with more
   // than one halo cell per thread and 1024-thread FP64 tiles keep one plane.
-  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + 1 : 1;
+  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + LA : 1;
   T hv[HD][HPT][NF];
 #pragma unroll
   for (int d = 0; d < HD; d++) {
@@ -151,9 +161,9 @@ __global__ void __launch_bounds__(TX * TY)
   for (int z = z0; z < z1; z++) {
     const int buf = (z - z0) & 1;
     {
-      const int off = amap(z + R + 1, k.n[2], k.guard[2], k.per[2]) * k.sz;
+      const int off = amap(z + R + LA, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
-      for (int f = 0; f < NF; f++) q[Q][f] = io.in[f][col + off];
+      for (int f = 0; f < NF; f++) q[Q + LA - 1][f] = io.in[f][col + off];
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) lds[buf][f][slot] = q[R][f];
@@ -201,7 +211,7 @@ __global__ void __launch_bounds__(TX * TY)
       stage_cell<T, FIRST, LAST, DIAG, STAR>(k, io, ai, aj, az, c, d, u, dp, As, Bs, ts);
     }
 #pragma unroll
-    for (int d = 0; d < Q; d++)
+    for (int d = 0; d < Q + LA - 1; d++)
 #pragma unroll
       for (int f = 0; f < NF; f++) q[d][f] = q[d + 1][f];
 #pragma unroll
@@ -247,16 +257,18 @@ __global__ void __launch_bounds__(TX * TY)
     hslot[h] = (hy + R) * LX + (hx + R);
   }
 
-  T q[Q + 1][NF];
+  // centre columns of planes z-R .. z+R+LA (see ez_stage_tiled); a second
+  // plane of look-ahead measured slower here (1.97 against 1.94 ms at 512^3)
+  constexpr int LA = 1;
+  T q[Q + LA][NF];
 #pragma unroll
-  for (int d = 0; d < Q; d++) {
+  for (int d = 0; d < Q + LA - 1; d++) {
     const int off = amap(z0 - R + d, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
     for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
   }
-  // halo of planes z .. z+HD-1, fetched with the centre columns (see
-  // ez_stage_tiled)
-  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + 1 : 1;
+  // halo of planes z .. z+HD-1, fetched with the centre columns
+  constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + LA : 1;
   T hv[HD][HPT][NF];
 #pragma unroll
   for (int d = 0; d < HD; d++) {
@@ -273,9 +285,9 @@ __global__ void __launch_bounds__(TX * TY)
   for (int z = z0; z < z1; z++) {
     const int buf = (z - z0) & 1;
     {
-      const int off = amap(z + R + 1, k.n[2], k.guard[2], k.per[2]) * k.sz;
+      const int off = amap(z + R + LA, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
-      for (int f = 0; f < NF; f++) q[Q][f] = io.in[f][col + off];
+      for (int f = 0; f < NF; f++) q[Q + LA - 1][f] = io.in[f][col + off];
     }
     const int az = z + k.guard[2];
     const int c = col + az * k.sz;
@@ -317,7 +329,7 @@ __global__ void __launch_bounds__(TX * TY)
       ko_cell_finish<T, STAR>(k, io, ai, aj, az, c, u, tko);
     }
 #pragma unroll
-    for (int d = 0; d < Q; d++)
+    for (int d = 0; d < Q + LA - 1; d++)
 #pragma unroll
       for (int f = 0; f < NF; f++) q[d][f] = q[d + 1][f];
     pc = pn;
