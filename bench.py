@@ -318,22 +318,29 @@ def run_unet_conv(args):
     print(json.dumps(out))
 
 
+def gr_wald_params(n: int, spin: float):
+    """Kerr-Schild Wald problem: box [-20, 20]^3 (M = 1), guard 3, absorbing
+    layer of 32 cells (sigpml 10, pmllen 1), dt = 0.4 delta."""
+    from coffeegpu_amd.params import SimParams
+
+    delta = 40.0 / n
+    return SimParams(dt=0.4 * delta, N=[n, n, n], guard=[3, 3, 3], lower=[-20.0] * 3,
+                     size=[40.0] * 3, nodes=[1, 1, 1], periodic_boundary=[False] * 3,
+                     pml=[32, 32, 32], pmllen=1, sigpml=10.0, divB_clean=True, ch2=1.0, tau=2.0,
+                     KOeps=0.1, use_edotb_damping=False, calc_current=True, clean_ep=True,
+                     check_egb=True, problem=0, a=spin)
+
+
 def run_gr(args):
     import torch
 
     from coffeegpu_amd import initial_data as ic
     from coffeegpu_amd.gr import FieldSolverGREZ
-    from coffeegpu_amd.params import SimParams
     from coffeegpu_amd.solver import SimData, SimEnvironment
 
     n = args.n if args.n != 512 else 384
     dtype = np.float32 if args.dtype == "fp32" else np.float64
-    delta = 40.0 / n
-    sp = SimParams(dt=0.4 * delta, N=[n, n, n], guard=[3, 3, 3], lower=[-20.0] * 3,
-                   size=[40.0] * 3, nodes=[1, 1, 1], periodic_boundary=[False] * 3,
-                   pml=[32, 32, 32], pmllen=1, sigpml=10.0, divB_clean=True, ch2=1.0, tau=2.0,
-                   KOeps=0.1, use_edotb_damping=False, calc_current=True, clean_ep=True,
-                   check_egb=True, problem=0, a=args.spin)
+    sp = gr_wald_params(n, args.spin)
     env = SimEnvironment(sp, dtype, "cuda:0")
     t0 = time.perf_counter()
     f = ic.wald_uniform(env.grid(), sp.a, noise=1e-3, seed=args.seed, dtype=dtype)
