@@ -373,6 +373,9 @@ def run_gr(args):
     bytes_per_launch = GR_SCALARS["gr_stage"] * esz * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
     W = solver.total_energy(data.B)
+    traffic = load_traffic(f"gr_stage_{n}_{args.dtype}_traffic.json")
+    if traffic and traffic.get("variant", 0) != args.variant:
+        traffic = None  # measured for another schedule
     out = {
         "metric": "cell-updates/s (Kerr-Schild GR Wald problem, 1 GPU)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -380,13 +383,13 @@ def run_gr(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32" if dtype == np.float32 else "f64",
         "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
         "config": {"workload": f"gr-wald {n}^3 box [-20,20]^3 a={sp.a}, pml 32 (sigpml 10, "
-                               "pmllen 1), horizon excision, FP64 metric on the fly",
+                               "pmllen 1), horizon excision, FP64 metric on the fly (halo "
+                               "points of stages 2-5 from a precomputed table)",
                    "cells": cells, "schedule": "tiled" if args.variant != 1 else "point+stage",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (load_traffic(f"gr_stage_{n}_{args.dtype}_traffic.json") or {}).get(
-                         "bytes_per_launch") if args.variant != 1 else None,
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": "gr_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernels": _kernel_table(timing, GR_SCALARS, esz, cells),

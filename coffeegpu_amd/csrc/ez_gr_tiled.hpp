@@ -25,6 +25,42 @@ __device__ __forceinline__ CksT<T> gr_metric_at(const GrK<T>& k, int ai, int aj,
   return cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak)));
 }
 
+// Metric of a point from the table io.W[0..6] (s, l_x, l_y, l_z, alpha,
+// sqrt(gamma), bu as gr_metric_table_kernel stores them; r is not stored and
+// not needed by gr_point_full).
+template <typename T>
+__device__ __forceinline__ CksT<T> gr_metric_load(const GrIO<T>& io, int c) {
+  CksT<T> m;
+  m.s = io.W[0][c];
+  m.l[0] = io.W[1][c];
+  m.l[1] = io.W[2][c];
+  m.l[2] = io.W[3][c];
+  m.alpha = io.W[4][c];
+  m.sg = io.W[5][c];
+  m.bfac = io.W[6][c];
+  m.r = 0.0;
+  return m;
+}
+
+// Fills the metric table io.W[0..6] at every point of the guarded grid with
+// the values gr_metric_at computes, so a kernel reading it sees bitwise the
+// same metric.
+template <typename T>
+__global__ void __launch_bounds__(256) gr_metric_table_kernel(const GrK<T> k, const GrIO<T> io) {
+  const int total = k.dims[0] * k.dims[1] * k.dims[2];
+  for (int c = blockIdx.x * 256 + (int)threadIdx.x; c < total; c += gridDim.x * 256) {
+    const int ai = c % k.dims[0], aj = (c / k.dims[0]) % k.dims[1], ak = c / k.sz;
+    const CksT<T> m = gr_metric_at(k, ai, aj, ak);
+    io.W[0][c] = m.s;
+    io.W[1][c] = m.l[0];
+    io.W[2][c] = m.l[1];
+    io.W[3][c] = m.l[2];
+    io.W[4][c] = m.alpha;
+    io.W[5][c] = m.sg;
+    io.W[6][c] = m.bfac;
+  }
+}
+
 // LDS slots: 0-2 E_i, 3-5 H_i, 6 sqrt(g) D^x, 7 sqrt(g) D^y, 8 sqrt(g) B^x,
 // 9 sqrt(g) B^y, 10 P.
 template <typename T>
@@ -62,7 +98,9 @@ __device__ __forceinline__ void gr_point_z(const CksT<T>& m, const T u[7], T v[7
   v[6] = u[6];
 }
 
-template <typename T, int TY, bool FIRST, bool LAST, bool DIAG>
+// MT: the halo points' metric is read from the table io.W[0..6] (one plane
+// ahead, with their state) instead of being computed in FP64 every plane.
+template <typename T, int TY, bool FIRST, bool LAST, bool DIAG, bool MT>
 __global__ void __launch_bounds__(64 * TY)
     gr_stage_tiled(const GrK<T> k, const GrIO<T> io, const T As, const T Bs, const TileMap tm) {
   constexpr int TX = 64, R = 2, LX = TX + 2 * R, LY = TY + 2 * R, NL = 11, NT = TX * TY;
@@ -129,13 +167,16 @@ __global__ void __launch_bounds__(64 * TY)
     for (int f = 0; f < 7; f++) inc[f] = io.in[f][col + ak * k.sz];
   }
   T hv[HPT][7];
+  CksT<T> hm[HPT];  // halo metric (MT)
   {
     const int ak = amap(z0, k.n[2], k.guard[2], 0);
 #pragma unroll
     for (int h = 0; h < HPT; h++)
-      if (hok[h])
+      if (hok[h]) {
 #pragma unroll
         for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + ak * k.sz];
+        if constexpr (MT) hm[h] = gr_metric_load(io, hcol[h] + ak * k.sz);
+      }
   }
   // dU (all but the first stage) and, in the last stage, Un of the current
   // plane, prefetched one plane ahead
@@ -177,7 +218,9 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
     for (int h = 0; h < HPT; h++)
       if (hok[h]) {
-        const CksT<T> m = gr_metric_at(k, hai[h], haj[h], akz);
+        CksT<T> m;
+        if constexpr (MT) m = hm[h];
+        else m = gr_metric_at(k, hai[h], haj[h], akz);
         T v[11];
         gr_point_full(m, hv[h], v);
 #pragma unroll
@@ -187,9 +230,11 @@ __global__ void __launch_bounds__(64 * TY)
       const int off = (akz + 1) * k.sz;
 #pragma unroll
       for (int h = 0; h < HPT; h++)
-        if (hok[h])
+        if (hok[h]) {
 #pragma unroll
           for (int f = 0; f < 7; f++) hv[h][f] = io.in[f][hcol[h] + off];
+          if constexpr (MT) hm[h] = gr_metric_load(io, hcol[h] + off);
+        }
     }
     T dn[7], un[7];
 #pragma unroll

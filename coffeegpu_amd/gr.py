@@ -46,7 +46,8 @@ class FieldSolverGREZ(FieldSolverEZ):
         self._diag = False
 
     def set_schedule(self, variant: int = 0, zchunk: int = 0):
-        """0 = LDS-tiled stage kernel computing E, H on the fly (default);
-        1 = point sweep (E, H arrays) + stage sweep.  Both give bitwise the
-        same results."""
+        """0 = LDS-tiled stage kernel computing E, H on the fly, the halo
+        points' metric of stages 2-5 read from a table filled once (default);
+        1 = point sweep (E, H arrays) + stage sweep; 2 = tiled with every
+        metric point computed on the fly.  All give bitwise the same results."""
         _lib.check(self._lib.coffee_set_schedule(self._ctx, int(variant), int(zchunk)), self._ctx)
