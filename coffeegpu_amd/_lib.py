@@ -48,7 +48,7 @@ class CoffeeParams(ctypes.Structure):
         ("pert_type", ctypes.c_int), ("a", ctypes.c_double), ("thp1", ctypes.c_double),
         ("thp2", ctypes.c_double), ("thp11", ctypes.c_double), ("thp21", ctypes.c_double),
         ("lower", _D3), ("size", _D3),
-        ("mask_lanes", ctypes.c_int),
+        ("mask_lanes", ctypes.c_int), ("shift_ghost", ctypes.c_int),
     ]
 
 

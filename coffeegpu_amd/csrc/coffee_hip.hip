@@ -92,6 +92,8 @@ int validate(coffee_ctx* c, const coffee_grid* g, const coffee_params* p) {
   if (total >= 2147483647.0) return fail(c, COFFEE_ERR_SIZE, "grid exceeds 2^31-1 cells per array");
   if (!(p->dt > 0)) return fail(c, COFFEE_ERR_INVALID, "dt must be positive");
   if (p->tau == 0) return fail(c, COFFEE_ERR_INVALID, "tau must be nonzero");
+  if (p->shift_ghost != 0)
+    return fail(c, COFFEE_ERR_INVALID, "shift_ghost != 0 is not supported (guards stay frozen)");
   return COFFEE_OK;
 }
 

@@ -122,14 +122,24 @@ struct SphState : SphStateBase {
   int graph_tile = -1;
 
   ~SphState() override {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     for (void* p : owned) (void)hipFree(p);
     if (d_runs) (void)hipFree(d_runs);
     if (d_items) (void)hipFree(d_items);
   }
 
+  // A captured graph holds the work-list pointer and the launch geometry of
+  // the tile it was captured with: any rebuild of the work list drops it.
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_tile = -1;
+  }
+
   int build_items(coffee_ctx* c) {
+    drop_graph();
     const Tile t = kTiles[tile];
     std::vector<SphItem> items;
     for (int r = 0; r < (int)h_runs.size(); r++)
@@ -205,10 +215,7 @@ struct SphState : SphStateBase {
   int step(coffee_ctx* c, hipStream_t s) override {
     if (!use_graph || c->timing || c->diagnostics) return step_direct(c, s);
     if (!graph_exec || graph_tile != tile) {
-      if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-      if (graph) (void)hipGraphDestroy(graph);
-      graph_exec = nullptr;
-      graph = nullptr;
+      drop_graph();
       hipStream_t cs;
       HIP_TRY(c, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       HIP_TRY(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -265,6 +272,8 @@ int make_run(coffee_ctx* c, SphState<T>* S, const coffee_grid& g, const coffee_p
   int rc = validate_sph(c, g);
   if (rc) return rc;
   if (!(p.dt > 0) || p.tau == 0) return fail(c, COFFEE_ERR_INVALID, "dt must be > 0 and tau != 0");
+  if (p.shift_ghost != 0)
+    return fail(c, COFFEE_ERR_INVALID, "shift_ghost != 0 is not supported (guards stay frozen)");
   std::memset(&R, 0, sizeof(R));
   R.dims0 = g.dims[0];
   R.dims1 = g.dims[1];
@@ -569,6 +578,13 @@ int autotune_impl(coffee_ctx* c, int trials, hipStream_t s, int* best, double* m
   return S->build_items(c);
 }
 
+// Every coffee_sph_batch_* entry takes only contexts made by
+// coffee_sph_batch_create: a single-run spherical context has no run arrays
+// until its first step, and other solvers hold other state.
+static bool is_batch(const coffee_ctx* c) {
+  return c && c->state && c->solver == COFFEE_SOLVER_EZ_SPHERICAL && c->is_batch;
+}
+
 extern "C" {
 
 int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid* grids,
@@ -581,6 +597,7 @@ int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid* grid
   if (!fields) return fail(nullptr, COFFEE_ERR_INVALID, "fields (E, B, P per run) must be given");
   coffee_ctx* c = new coffee_ctx();
   c->solver = COFFEE_SOLVER_EZ_SPHERICAL;
+  c->is_batch = 1;
   c->scalar = scalar_bytes;
   c->grid = grids[0];
   c->params = params[0];
@@ -596,11 +613,13 @@ int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid* grid
   return COFFEE_OK;
 }
 
-int coffee_sph_batch_destroy(coffee_batch* b) { return coffee_ctx_destroy(b); }
+int coffee_sph_batch_destroy(coffee_batch* b) {
+  if (b && !is_batch(b)) return fail(b, COFFEE_ERR_STATE, "not a spherical batch");
+  return coffee_ctx_destroy(b);
+}
 
 int coffee_sph_batch_step(coffee_batch* c, int nsteps, void* stream) {
-  if (!c || !c->state || c->solver != COFFEE_SOLVER_EZ_SPHERICAL)
-    return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
   auto* S = static_cast<SphStateBase*>(c->state);
   for (int k = 0; k < nsteps; k++) {
     int rc = S->step(c, (hipStream_t)stream);
@@ -610,7 +629,7 @@ int coffee_sph_batch_step(coffee_batch* c, int nsteps, void* stream) {
 }
 
 int coffee_sph_batch_set_tile(coffee_batch* c, int tile_index) {
-  if (!c || !c->state) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
   if (tile_index < 0 || tile_index >= kNumTiles) return fail(c, COFFEE_ERR_INVALID, "bad tile index");
   if (c->scalar == 4) {
     auto* S = static_cast<SphState<float>*>(c->state);
@@ -634,30 +653,29 @@ int coffee_sph_num_tiles(void) { return kNumTiles; }
 
 int coffee_sph_batch_autotune(coffee_batch* c, int trials, void* stream, int* best_tile,
                               double* ms_per_tile) {
-  if (!c || !c->state || c->solver != COFFEE_SOLVER_EZ_SPHERICAL)
-    return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
   hipStream_t s = (hipStream_t)stream;
   return c->scalar == 4 ? autotune_impl<float>(c, trials, s, best_tile, ms_per_tile)
                         : autotune_impl<double>(c, trials, s, best_tile, ms_per_tile);
 }
 
 int coffee_sph_batch_use_graph(coffee_batch* c, int enable) {
-  if (!c || !c->state) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
   if (c->scalar == 4) static_cast<SphState<float>*>(c->state)->use_graph = enable != 0;
   else static_cast<SphState<double>*>(c->state)->use_graph = enable != 0;
   return COFFEE_OK;
 }
 
 int coffee_sph_batch_luminosity(coffee_batch* c, const double* radius, double* out, void* stream) {
-  if (!c || !c->state || c->solver != COFFEE_SOLVER_EZ_SPHERICAL || !radius || !out)
-    return fail(c, COFFEE_ERR_INVALID, "not a spherical batch or null argument");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!radius || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
   return c->scalar == 4 ? luminosity_impl<float>(c, radius, out, s)
                         : luminosity_impl<double>(c, radius, out, s);
 }
 
 int coffee_sph_batch_status(coffee_batch* c, int* active, double* t, int* step) {
-  if (!c || !c->state) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
   HIP_TRY(c, hipDeviceSynchronize());
   if (c->scalar == 4) {
     auto* S = static_cast<SphState<float>*>(c->state);

@@ -30,6 +30,7 @@ std::string& create_error();
 
 struct coffee_ctx {
   int solver = 0;
+  int is_batch = 0;  // made by coffee_sph_batch_create
   int scalar = 8;
   int device = 0;
   coffee_grid grid{};

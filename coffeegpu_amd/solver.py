@@ -79,11 +79,13 @@ class MultiArray:
         self.dev.copy_(other.dev)
 
     def sync_to_host(self) -> np.ndarray:
-        """Copies the device buffer into the host mirror and returns it.  The
-        mirror is one page-locked buffer kept for the array's lifetime (as
-        multi_array keeps its host copy), so the next sync_to_host overwrites
-        it; page-locked copies run at the link rate (≈57 GB/s on MI355X
-        against ≈3 GB/s into freshly allocated pageable memory)."""
+        """Copies the device buffer into the host mirror and returns a numpy
+        VIEW of it (no copy).  The mirror is one page-locked buffer kept for
+        the array's lifetime (as multi_array keeps its host copy): the next
+        sync_to_host of this array overwrites the data every earlier returned
+        view shows, so a caller that keeps snapshots must `.copy()` them.
+        Page-locked copies run at the link rate (≈57 GB/s on MI355X against
+        ≈3 GB/s into freshly allocated pageable memory)."""
         torch = _torch()
         if self._pinned is None:
             self._pinned = torch.empty(self.dev.shape, dtype=self.dev.dtype,

@@ -78,6 +78,7 @@ inline coffee_params make_coffee_params(const sim_params& p, int mask_lanes) {
   c.pert_type = p.pert_type;
   c.a = p.a;
   c.mask_lanes = mask_lanes;
+  c.shift_ghost = p.shift_ghost;
   return c;
 }
 

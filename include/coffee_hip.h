@@ -82,6 +82,10 @@ typedef struct coffee_params {
    * build's vector width (16 for FP32, 8 for FP64 on AVX-512) for parity with
    * it; 1 evaluates the mask at the node. */
   int mask_lanes;
+  /* sim_params::shift_ghost: guard layers the reference's flat CPU loops and
+   * GR kernels also update (field_solver_EZ.cpp:128-136).  Only 0 is
+   * supported; other values are rejected with COFFEE_ERR_INVALID. */
+  int shift_ghost;
 } coffee_params;
 
 /* Device arrays of one solver call: the reference's sim_data members and

@@ -52,3 +52,69 @@ def test_abi_version_and_create_error_without_gpu():
     rc = lib.coffee_ctx_create(0, 3, ctypes.byref(g), ctypes.byref(p), ctypes.byref(ctx))
     assert rc == -1
     assert b"scalar_bytes" in lib.coffee_last_error(None)
+
+
+def _flat_grid_params(n=(16, 16, 16), guard=3):
+    from coffeegpu_amd import _lib
+
+    g = _lib.CoffeeGrid()
+    p = _lib.CoffeeParams()
+    for a in range(3):
+        g.dims[a] = n[a] + 2 * guard
+        g.guard[a] = guard
+        g.delta[a] = 1.0 / n[a]
+        g.sizes[a] = 1.0
+        p.periodic[a] = 1
+        p.size[a] = 1.0
+    p.dt = 0.4 / n[0]
+    p.tau = 2.0
+    return g, p
+
+
+def test_shift_ghost_rejected_before_any_device_call():
+    """sim_params::shift_ghost > 0 (guard layers the reference also updates,
+    field_solver_EZ.cpp:128-136) is not supported: the context is refused
+    instead of running with frozen guards."""
+    from coffeegpu_amd import _lib
+
+    lib = _lib.load()
+    g, p = _flat_grid_params()
+    p.shift_ghost = 1
+    ctx = ctypes.c_void_p()
+    for solver in (_lib.COFFEE_SOLVER_EZ, _lib.COFFEE_SOLVER_GR_EZ):
+        rc = lib.coffee_ctx_create(solver, 4, ctypes.byref(g), ctypes.byref(p), ctypes.byref(ctx))
+        assert rc == -1
+        assert b"shift_ghost" in lib.coffee_last_error(None)
+        assert not ctx.value
+
+
+def test_shift_ghost_rejected_by_python_mirror_params():
+    from coffeegpu_amd.params import SimParams
+    from coffeegpu_amd.solver import coffee_params
+
+    p = coffee_params(SimParams(shift_ghost=2))
+    assert p.shift_ghost == 2  # carried to the ABI, which refuses it
+
+
+def test_oversized_grid_returns_err_size():
+    """A grid of 2^31 or more cells per array does not fit the kernels' 32-bit
+    cell index: COFFEE_ERR_SIZE, before any device call."""
+    from coffeegpu_amd import _lib
+
+    lib = _lib.load()
+    g, p = _flat_grid_params(n=(1290, 1290, 1290))
+    ctx = ctypes.c_void_p()
+    rc = lib.coffee_ctx_create(0, 4, ctypes.byref(g), ctypes.byref(p), ctypes.byref(ctx))
+    assert rc == -3
+    assert b"2^31" in lib.coffee_last_error(None)
+    assert not ctx.value
+
+
+def test_batch_entries_refuse_null_context():
+    from coffeegpu_amd import _lib
+
+    lib = _lib.load()
+    assert lib.coffee_sph_batch_step(None, 1, None) == -4
+    assert lib.coffee_sph_batch_set_tile(None, 0) == -4
+    assert lib.coffee_sph_batch_use_graph(None, 1) == -4
+    assert lib.coffee_sph_batch_autotune(None, 1, None, None, None) == -4
