@@ -18,8 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcoffee_hip.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip", "unet.hip", "coffee_io.hip",
-               "coffee_noalloc.hip"]
+HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip", "unet.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 # -ffp-contract=off: every fused multiply-add is written out as fma() in the

@@ -20,6 +20,10 @@
 #include <string.h>
 #include "ffe_oracle.h"
 
+int ffe_plant = FFE_PLANT_NONE;
+void ffe_set_plant(int which) { ffe_plant = (which > 0 && which < FFE_PLANT_COUNT) ? which : 0; }
+int ffe_get_plant(void) { return ffe_plant; }
+
 #define REAL double
 #define SFX _f64
 #define DIAG_T ffe_diag_f64

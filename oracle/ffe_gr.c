@@ -9,13 +9,26 @@
  * of div4, .cu:44-88).
  *
  * Parity of this restatement with the CUDA source is unpinned by the
- * reference itself (no runnable build).  It is pinned instead by
- * tests/test_oracle_gr.py: metric identities (gamma_ij gamma^jk = delta,
- * det gamma_ij = gamma, alpha sqrt(gamma) = 1, beta^i = gamma^ij beta_j, the
- * a = 0 Schwarzschild form), the constitutive relations (E.B = alpha D_i B^i),
- * and stationarity of the exact vacuum Wald solution (built from its vector
- * potential by complex-step differentiation): one RK stage changes it only by
- * the 4th-order truncation error, converging at order 4 under refinement.
+ * reference itself (no runnable build).  It is pinned instead by:
+ *   tests/test_oracle_gr.py
+ *     - cks_at / cks_r: metric identities (gamma_ij gamma^jk = delta,
+ *       det gamma_ij = gamma, alpha sqrt(gamma) = 1, beta^i = gamma^ij beta_j)
+ *       and the a = 0 Schwarzschild form;
+ *     - gr_eh: the constitutive relations (E.B = alpha D_i B^i, ...);
+ *     - gr_rk1 (vacuum part): the exact Wald solution is stationary to 4th
+ *       order; a wrong spin is not;
+ *     - gr_absorb: cells with r < 0.01 r_H are zeroed;
+ *   tests/test_oracle_pins_gr.py
+ *     - gr_eh + gr_rk1 + div4 (curl, div, current, cleaning with its tanh
+ *       weight, dP): one stage's dU/dt on manufactured D, B, P with E.B != 0,
+ *       div D != 0, P != 0 converges at 4th order to the 3+1 force-free
+ *       right-hand side built from the Kerr-Schild 4-metric's own definition
+ *       (complex-step derivatives), at a = 0, 0.7, 0.99;
+ *     - gr_project: closed form with metric dot products, D.B = 0,
+ *       D^2 <= B^2, idempotence;
+ *     - gr_absorb: closed form of the outer layer and the horizon excision;
+ *     - gr_ko: checkerboard decay by (1 - 3 KOeps), linear fields untouched;
+ *     - each planted mistake (FFE_PLANT_*) fails one of these.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -120,29 +133,33 @@ static inline double diff6_2(const double *f, long ijk, long s) {
           15.0 * f[ijk + s] - 6.0 * f[ijk + 2 * s] + f[ijk + 3 * s]);
 }
 
-/* div4 (.cu:44-88) */
+/* div4 (.cu:44-88).  SGT: sqrt(gamma) at a tap (at the centre under the
+ * DIV_CENTRE planted mistake). */
+#define SGT(xx, yy, zz) \
+  (ffe_plant == FFE_PLANT_DIV_CENTRE ? cks_sg(a, x, y, z) : cks_sg(a, xx, yy, zz))
 static double div4(const ggrid_t *g, double a, const double *fx, const double *fy,
                    const double *fz, long ijk, double x, double y, double z) {
   const double d0 = g->delta[0], d1 = g->delta[1], d2 = g->delta[2];
-  double tx = (fx[ijk - 2] * cks_sg(a, x - 2.0 * d0, y, z) -
-               8.0 * fx[ijk - 1] * cks_sg(a, x - 1.0 * d0, y, z) +
-               8.0 * fx[ijk + 1] * cks_sg(a, x + 1.0 * d0, y, z) -
-               fx[ijk + 2] * cks_sg(a, x + 2.0 * d0, y, z)) /
+  double tx = (fx[ijk - 2] * SGT(x - 2.0 * d0, y, z) -
+               8.0 * fx[ijk - 1] * SGT(x - 1.0 * d0, y, z) +
+               8.0 * fx[ijk + 1] * SGT(x + 1.0 * d0, y, z) -
+               fx[ijk + 2] * SGT(x + 2.0 * d0, y, z)) /
               12.0 / d0;
   long s = g->sy;
-  double ty = (fy[ijk - 2 * s] * cks_sg(a, x, y - 2.0 * d1, z) -
-               8.0 * fy[ijk - 1 * s] * cks_sg(a, x, y - 1.0 * d1, z) +
-               8.0 * fy[ijk + 1 * s] * cks_sg(a, x, y + 1.0 * d1, z) -
-               fy[ijk + 2 * s] * cks_sg(a, x, y + 2.0 * d1, z)) /
+  double ty = (fy[ijk - 2 * s] * SGT(x, y - 2.0 * d1, z) -
+               8.0 * fy[ijk - 1 * s] * SGT(x, y - 1.0 * d1, z) +
+               8.0 * fy[ijk + 1 * s] * SGT(x, y + 1.0 * d1, z) -
+               fy[ijk + 2 * s] * SGT(x, y + 2.0 * d1, z)) /
               12.0 / d1;
   s = g->sz;
-  double tz = (fz[ijk - 2 * s] * cks_sg(a, x, y, z - 2.0 * d2) -
-               8.0 * fz[ijk - 1 * s] * cks_sg(a, x, y, z - 1.0 * d2) +
-               8.0 * fz[ijk + 1 * s] * cks_sg(a, x, y, z + 1.0 * d2) -
-               fz[ijk + 2 * s] * cks_sg(a, x, y, z + 2.0 * d2)) /
+  double tz = (fz[ijk - 2 * s] * SGT(x, y, z - 2.0 * d2) -
+               8.0 * fz[ijk - 1 * s] * SGT(x, y, z - 1.0 * d2) +
+               8.0 * fz[ijk + 1 * s] * SGT(x, y, z + 1.0 * d2) -
+               fz[ijk + 2 * s] * SGT(x, y, z + 2.0 * d2)) /
               12.0 / d2;
   return (tx + ty + tz) / cks_sg(a, x, y, z);
 }
+#undef SGT
 
 #define ALL_CELLS(g)                                            \
   _Pragma("omp parallel for schedule(static)")                  \
@@ -194,17 +211,19 @@ static void gr_rk1(const ffe_params *p, const ggrid_t *g, double *const U[7],
     cks_t m;
     cks_at(a, x, y, z, &m);
     const double r = m.r, gmsqrt = m.sg, alpha = m.alpha;
+    const double gmr = ffe_plant == FFE_PLANT_ROT_NO_SQRTG ? 1.0 : gmsqrt;
+    const double gmj = ffe_plant == FFE_PLANT_J_NO_SQRTG ? 1.0 : gmsqrt;
     const double *Ex = EH[0], *Ey = EH[1], *Ez = EH[2], *Hx = EH[3], *Hy = EH[4], *Hz = EH[5];
     const double *Dx = U[0], *Dy = U[1], *Dz = U[2], *Bx = U[3], *By = U[4], *Bz = U[5];
     const double *P = U[6];
     const double i0 = g->inv_delta[0], i1 = g->inv_delta[1], i2 = g->inv_delta[2];
 
-    double rotHx = (df1(Hz, ijk, sy, i1) - df1(Hy, ijk, sz, i2)) / gmsqrt;
-    double rotHy = (df1(Hx, ijk, sz, i2) - df1(Hz, ijk, 1, i0)) / gmsqrt;
-    double rotHz = (df1(Hy, ijk, 1, i0) - df1(Hx, ijk, sy, i1)) / gmsqrt;
-    double rotEx = (df1(Ez, ijk, sy, i1) - df1(Ey, ijk, sz, i2)) / gmsqrt;
-    double rotEy = (df1(Ex, ijk, sz, i2) - df1(Ez, ijk, 1, i0)) / gmsqrt;
-    double rotEz = (df1(Ey, ijk, 1, i0) - df1(Ex, ijk, sy, i1)) / gmsqrt;
+    double rotHx = (df1(Hz, ijk, sy, i1) - df1(Hy, ijk, sz, i2)) / gmr;
+    double rotHy = (df1(Hx, ijk, sz, i2) - df1(Hz, ijk, 1, i0)) / gmr;
+    double rotHz = (df1(Hy, ijk, 1, i0) - df1(Hx, ijk, sy, i1)) / gmr;
+    double rotEx = (df1(Ez, ijk, sy, i1) - df1(Ey, ijk, sz, i2)) / gmr;
+    double rotEy = (df1(Ex, ijk, sz, i2) - df1(Ez, ijk, 1, i0)) / gmr;
+    double rotEz = (df1(Ey, ijk, 1, i0) - df1(Ex, ijk, sy, i1)) / gmr;
 
     double divD = div4(g, a, Dx, Dy, Dz, ijk, x, y, z);
     double divB = div4(g, a, Bx, By, Bz, ijk, x, y, z);
@@ -221,9 +240,14 @@ static void gr_rk1(const ffe_params *p, const ggrid_t *g, double *const U[7],
       double B2 = Bx[ijk] * Bdx + By[ijk] * Bdy + Bz[ijk] * Bdz;
       if (B2 < TINY_G) B2 = TINY_G;
       double Jp = (Bdx * rotHx + Bdy * rotHy + Bdz * rotHz) - (Ddx * rotEx + Ddy * rotEy + Ddz * rotEz);
-      Jx = (divD * (Ey[ijk] * Bdz - Ez[ijk] * Bdy) / gmsqrt + Jp * Bx[ijk]) / B2;
-      Jy = (divD * (Ez[ijk] * Bdx - Ex[ijk] * Bdz) / gmsqrt + Jp * By[ijk]) / B2;
-      Jz = (divD * (Ex[ijk] * Bdy - Ey[ijk] * Bdx) / gmsqrt + Jp * Bz[ijk]) / B2;
+      Jx = (divD * (Ey[ijk] * Bdz - Ez[ijk] * Bdy) / gmj + Jp * Bx[ijk]) / B2;
+      Jy = (divD * (Ez[ijk] * Bdx - Ex[ijk] * Bdz) / gmj + Jp * By[ijk]) / B2;
+      Jz = (divD * (Ex[ijk] * Bdy - Ey[ijk] * Bdx) / gmj + Jp * Bz[ijk]) / B2;
+      if (ffe_plant == FFE_PLANT_J_SIGN) {
+        Jx = -Jx;
+        Jy = -Jy;
+        Jz = -Jz;
+      }
     } else {
       Jx = 0.0;
       Jy = 0.0;
@@ -236,9 +260,15 @@ static void gr_rk1(const ffe_params *p, const ggrid_t *g, double *const U[7],
     double Pxu = m.gu11 * Pxd + m.gu12 * Pyd + m.gu13 * Pzd;
     double Pyu = m.gu12 * Pxd + m.gu22 * Pyd + m.gu23 * Pzd;
     double Pzu = m.gu13 * Pxd + m.gu23 * Pyd + m.gu33 * Pzd;
+    if (ffe_plant == FFE_PLANT_GRADP_LOWER) {
+      Pxu = m.gd11 * Pxd + m.gd12 * Pyd + m.gd13 * Pzd;
+      Pyu = m.gd12 * Pxd + m.gd22 * Pyd + m.gd23 * Pzd;
+      Pzu = m.gd13 * Pxd + m.gd23 * Pyd + m.gd33 * Pzd;
+    }
 
     double r0 = 5.0, del = 1.0;
     double shape = 0.5 * (1.0 - tanh((r - r0) / del));
+    if (ffe_plant == FFE_PLANT_CLEAN_SHAPE) shape = 0.5 * (1.0 + tanh((r - r0) / del));
 
     if (p->divB_clean) {
       dU[3][ijk] = As * dU[3][ijk] + dt * (-rotEx + (-alpha * Pxu + m.bu1 * divB) * shape);
@@ -254,7 +284,7 @@ static void gr_rk1(const ffe_params *p, const ggrid_t *g, double *const U[7],
     dU[2][ijk] = As * dU[2][ijk] + dt * (rotHz - Jz);
     dU[6][ijk] = As * dU[6][ijk] +
                  dt * (-alpha * divB + m.bu1 * Pxd + m.bu2 * Pyd + m.bu3 * Pzd) -
-                 alpha * P[ijk] / tau;
+                 alpha * (ffe_plant == FFE_PLANT_TAU_DT ? dt * P[ijk] : P[ijk]) / tau;
     if (diag) {
       if (diag->J[0]) diag->J[0][ijk] = Jx;
       if (diag->J[1]) diag->J[1][ijk] = Jy;
@@ -282,6 +312,10 @@ static void gr_project(const ffe_params *p, const ggrid_t *g, double *const U[7]
     double x = gpos(g, 0, i), y = gpos(g, 1, j), z = gpos(g, 2, k);
     cks_t m;
     cks_at(a, x, y, z, &m);
+    if (ffe_plant == FFE_PLANT_PROJ_FLAT) {
+      m.gd11 = m.gd22 = m.gd33 = 1.0;
+      m.gd12 = m.gd13 = m.gd23 = 0.0;
+    }
     double *Dx = U[0], *Dy = U[1], *Dz = U[2];
     const double *Bx = U[3], *By = U[4], *Bz = U[5];
     if (p->clean_ep) {
@@ -315,8 +349,9 @@ static void gr_project(const ffe_params *p, const ggrid_t *g, double *const U[7]
 
 /* pmlsigma (boundary.cu:7-15) */
 static inline double pmlsigma(double x, double xl, double xh, double pmlscale, double sig0) {
-  if (x > xh) return sig0 * cube((x - xh) / pmlscale);
-  if (x < xl) return sig0 * cube((xl - x) / pmlscale);
+  const int sq_only = ffe_plant == FFE_PLANT_SIGMA_SQUARED;
+  if (x > xh) return sig0 * (sq_only ? sq((x - xh) / pmlscale) : cube((x - xh) / pmlscale));
+  if (x < xl) return sig0 * (sq_only ? sq((xl - x) / pmlscale) : cube((xl - x) / pmlscale));
   return 0.0;
 }
 
@@ -351,7 +386,7 @@ static void gr_absorb(const ffe_params *p, const ggrid_t *g, double *const U[7],
     double x = gpos(g, 0, i), y = gpos(g, 1, j), z = gpos(g, 2, k);
     double r = cks_r(a, x, y, z);
     if (r < r1) {
-      double sig = sig0 * cube((r1 - r) / dd) * dt;
+      double sig = sig0 * cube((r1 - r) / dd) * (ffe_plant == FFE_PLANT_EXCISION_NODT ? 1.0 : dt);
       if (sig > TINY_G) {
         for (int c = 0; c < 6; c++)
           U[c][ijk] = exp(-sig) * Un[c][ijk] + (1.0 - exp(-sig)) / sig * (U[c][ijk] - Un[c][ijk]);
@@ -430,5 +465,29 @@ int ffe_gr_eh_f64(const ffe_params *p, double *const U[7], double *const EH[6]) 
   ggrid_t g;
   make_ggrid(p, &g);
   gr_eh(&g, p->a, U, EH);
+  return 0;
+}
+
+/* The projections alone, for tests. */
+int ffe_gr_project_f64(const ffe_params *p, double *const U[7]) {
+  ggrid_t g;
+  make_ggrid(p, &g);
+  gr_project(p, &g, U);
+  return 0;
+}
+
+/* The outer layer and horizon excision alone, for tests. */
+int ffe_gr_absorb_f64(const ffe_params *p, double *const U[7], double *const Un[7]) {
+  ggrid_t g;
+  make_ggrid(p, &g);
+  gr_absorb(p, &g, U, Un);
+  return 0;
+}
+
+/* Kreiss-Oliger alone (tmp: 7 scratch arrays), for tests. */
+int ffe_gr_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7]) {
+  ggrid_t g;
+  make_ggrid(p, &g);
+  gr_ko(p, &g, U, tmp);
   return 0;
 }

@@ -84,6 +84,31 @@ typedef struct ffe_diag_f32 {
   float *dU_KO_cum, *dU_Epar_cum, *dU_EgtB_cum;
 } ffe_diag_f32;
 
+/* ---- planted mistakes ------------------------------------------------------
+ * A test sets one of these to switch a single deliberate error into the
+ * spherical / Kerr-Schild restatement and shows that its check then fails
+ * (mutation check of the test itself).  0 = none, the only value outside such
+ * tests; with 0 every result is bitwise what the unmodified restatement
+ * computes. */
+enum {
+  FFE_PLANT_NONE = 0,
+  FFE_PLANT_J_SIGN = 1,         /* force-free current with the wrong sign */
+  FFE_PLANT_J_NO_SQRTG = 2,     /* rho E x B term without its 1/sqrt(gamma) */
+  FFE_PLANT_GRADP_LOWER = 3,    /* grad P raised with gamma_ij instead of gamma^ij */
+  FFE_PLANT_TAU_DT = 4,         /* P damping rate P/tau taken per unit time, not per step */
+  FFE_PLANT_SIGMA_SQUARED = 5,  /* absorbing-layer sigma ~ depth^2 instead of depth^3 */
+  FFE_PLANT_PROJ_FLAT = 6,      /* E.B and |E|,|B| of the projections without the metric */
+  FFE_PLANT_ROT_NO_SQRTG = 7,   /* curl without its 1/sqrt(gamma) */
+  FFE_PLANT_DIV_CENTRE = 8,     /* divergence with sqrt(gamma) of the centre, not the taps */
+  FFE_PLANT_STAR_SIGN = 9,      /* corotation field E = +(Omega x r) x B inside the star */
+  FFE_PLANT_CLEAN_SHAPE = 10,   /* Kerr-Schild cleaning weight 1/2(1 + tanh) */
+  FFE_PLANT_EXCISION_NODT = 11, /* horizon excision sigma without its factor dt */
+  FFE_PLANT_COUNT = 12
+};
+void ffe_set_plant(int which);
+int ffe_get_plant(void);
+extern int ffe_plant;
+
 /* ---- flat Cartesian EZ (field_solver_EZ.cpp) ---------------------------- */
 
 /* Advance nsteps steps from `time`.  U: 7 state arrays; dU: 7 LSRK registers
@@ -126,6 +151,11 @@ int ffe_sph_axis_f64(const ffe_params *p, double *const U[7]);
 int ffe_sph_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7]);
 int ffe_sph_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
                          double *const L[6], double As, double Bs);
+/* kernel_Epar_sph + kernel_EgtB_sph, kernel_boundary_pulsar_sph at time t,
+ * kernel_boundary_absorbing_sph (Un: step-start copy), interior cells. */
+int ffe_sph_project_f64(const ffe_params *p, double *const U[7]);
+int ffe_sph_boundary_f64(const ffe_params *p, double *const U[7], double t);
+int ffe_sph_absorb_f64(const ffe_params *p, double *const U[7], double *const Un[7]);
 
 /* ---- Kerr-Schild GR EZ (field_solver_gr_EZ.cu), FP64 ------------------- */
 
@@ -138,6 +168,11 @@ int ffe_gr_evolve_f64(const ffe_params *p, double *const U[7], double *const dU[
 int ffe_gr_rk_stage_f64(const ffe_params *p, double *const U[7], double *const dU[7],
                         double *const EH[6], const ffe_diag_f64 *diag, double As, double Bs);
 int ffe_gr_eh_f64(const ffe_params *p, double *const U[7], double *const EH[6]);
+/* kernel_clean_epar_gr + kernel_check_eGTb_gr; the outer layer and horizon
+ * excision of boundary_absorbing (Un = Dtmp, Btmp), interior cells. */
+int ffe_gr_project_f64(const ffe_params *p, double *const U[7]);
+int ffe_gr_absorb_f64(const ffe_params *p, double *const U[7], double *const Un[7]);
+int ffe_gr_ko_f64(const ffe_params *p, double *const U[7], double *const tmp[7]);
 /* gd11 gd12 gd13 gd22 gd23 gd33, gu11 .. gu33, bd1..3, bu1..3, alpha,
  * sqrt(gamma), r at one point (metric_cks_impl.hpp). */
 void ffe_gr_metric_f64(double a, double x, double y, double z, double out[21]);

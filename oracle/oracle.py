@@ -311,3 +311,102 @@ def gr_metric(a: float, x: float, y: float, z: float) -> dict:
     L.ffe_gr_metric_f64(ctypes.c_double(a), ctypes.c_double(x), ctypes.c_double(y),
                         ctypes.c_double(z), out)
     return dict(zip(GR_METRIC_NAMES, list(out)))
+
+
+# ---- building blocks and planted mistakes (pinning tests) ------------------------
+
+PLANTS = {
+    "J_SIGN": 1, "J_NO_SQRTG": 2, "GRADP_LOWER": 3, "TAU_DT": 4, "SIGMA_SQUARED": 5,
+    "PROJ_FLAT": 6, "ROT_NO_SQRTG": 7, "DIV_CENTRE": 8, "STAR_SIGN": 9, "CLEAN_SHAPE": 10,
+    "EXCISION_NODT": 11,
+}
+
+
+class planted:
+    """Context manager switching one deliberate mistake (PLANTS) into the
+    spherical / Kerr-Schild restatement, for mutation checks of the tests."""
+
+    def __init__(self, name: Optional[str]):
+        self.code = PLANTS[name] if name else 0
+
+    def __enter__(self):
+        lib().ffe_set_plant(ctypes.c_int(self.code))
+        return self
+
+    def __exit__(self, *exc):
+        lib().ffe_set_plant(ctypes.c_int(0))
+        return False
+
+
+def _f64_ptrs(U):
+    return _ptrs([np.ascontiguousarray(u) for u in U], ctypes.c_double)
+
+
+def sph_stage_rhs(sp, U: np.ndarray) -> np.ndarray:
+    """dU/dt of one stage (As = 0, Bs = 0: U is left unchanged), FP64."""
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    dU = np.zeros_like(U)
+    Lw = np.zeros((6,) + U.shape[1:])
+    rc = lib().ffe_sph_rk_stage_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                                    _ptrs(list(dU), ctypes.c_double),
+                                    _ptrs(list(Lw), ctypes.c_double), ctypes.c_double(0.0),
+                                    ctypes.c_double(0.0))
+    assert rc == 0
+    return dU / sp.dt
+
+
+def sph_project(sp, U: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    lib().ffe_sph_project_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double))
+    return U
+
+
+def sph_boundary(sp, U: np.ndarray, t: float) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    lib().ffe_sph_boundary_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                               ctypes.c_double(t))
+    return U
+
+
+def sph_absorb(sp, U: np.ndarray, Un: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    Un = np.ascontiguousarray(Un, dtype=np.float64).copy()
+    lib().ffe_sph_absorb_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                             _ptrs(list(Un), ctypes.c_double))
+    return U
+
+
+def gr_stage_rhs(sp, U: np.ndarray) -> np.ndarray:
+    st = GrState(U)
+    gr_rk_stage(sp, st, 0.0, 0.0)
+    return st.dU / sp.dt
+
+
+def gr_project(sp, U: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    lib().ffe_gr_project_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double))
+    return U
+
+
+def gr_ko(sp, U: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    tmp = np.zeros_like(U)
+    lib().ffe_gr_ko_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                        _ptrs(list(tmp), ctypes.c_double))
+    return U
+
+
+def sph_ko(sp, U: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    tmp = np.zeros_like(U)
+    lib().ffe_sph_ko_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                         _ptrs(list(tmp), ctypes.c_double))
+    return U
+
+
+def gr_absorb(sp, U: np.ndarray, Un: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    Un = np.ascontiguousarray(Un, dtype=np.float64).copy()
+    lib().ffe_gr_absorb_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
+                            _ptrs(list(Un), ctypes.c_double))
+    return U
