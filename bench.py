@@ -49,11 +49,19 @@ def parse():
     ap.add_argument("--dtype", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the extra keys (attainable bandwidth, "
+                         "batched sweep, retiring sweep, Kerr-Schild Wald)")
+    ap.add_argument("--extra-steps", type=int, default=10,
+                    help="timed steps of the sweep and Kerr-Schild extras")
+    ap.add_argument("--retire-scale", type=float, default=5e-4,
+                    help="retiring sweep: fraction of the steps of 5 rotation periods")
     ap.add_argument("--variant", type=int, default=0,
                     help="0 = LDS-tiled (default tile), 1 = simple, 2-4 = 64-wide tiles of "
                          "4/8/16 rows, 5/6 = 128/256-wide tiles, 7 = 128 x 4")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0 = auto)")
-    ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--cpu-sample-n", type=int, default=0,
+                    help="CPU baseline sample size (0: 256 with several MPI ranks, else 128)")
     ap.add_argument("--cpu-sample-steps", type=int, default=10)
     ap.add_argument("--workload", choices=["alfven3d", "sph-sweep", "gr-wald", "unet-conv"],
                     default="alfven3d",
@@ -106,31 +114,43 @@ def load_traffic(name: str):
 
 
 def cpu_baseline(args, dtype):
-    """Reference CPU build (oracle/_ref, one MPI rank = one core) on a bounded
-    sample of the same workload; falls back to the oracle port (OpenMP)."""
+    """Reference CPU build (oracle/_ref) on a bounded sample of the same
+    workload.  The reference is single-threaded per MPI rank and scales with
+    ranks, so it runs under mpiexec with one rank per core (up to 16, the z
+    axis split over the ranks, nodes = [1, 1, ranks]); with no mpiexec it runs
+    as one rank; with no reference build, the oracle port (OpenMP) is timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from coffeegpu_amd import initial_data as ic
     from coffeegpu_amd.params import Grid
 
-    n = args.cpu_sample_n
+    ranks = min(16, os.cpu_count() or 1) if O.mpiexec_available() else 1
+    n = args.cpu_sample_n if args.cpu_sample_n > 0 else (256 if ranks > 1 else 128)
+    while ranks > 1 and n // ranks < 3:  # every rank needs at least `guard` planes
+        ranks //= 2
     sp = alfven_params(n, dtype)
     g = Grid.from_params(sp, dtype)
     fields, _ = ic.alfven_packets_seeded(g, args.seed, dtype=dtype)
     U0 = ic.stack(fields)
+    del fields
     steps = args.cpu_sample_steps
     sample = f"{n}^3 cells, {steps} steps after 2 warm-up steps, same physics and dtype"
     if O.reference_available(dtype):
-        try:
-            _, _, err = O.run_reference(sp, U0, steps + 2, timing_warmup=2)
-            for line in err.splitlines():
-                if line.startswith("TIMING"):
-                    kv = dict(t.split("=") for t in line.split()[1:])
-                    return {"value": float(kv["cell_updates_per_s"]), "unit": "cell-updates/s",
-                            "cores": 1, "kind": "reference",
-                            "sample": sample + "; reference CPU build (use_cuda=0), 1 MPI rank"}
-        except Exception as e:  # reference binary unusable on this host
-            print(f"[bench] reference CPU build unavailable here: {e}", file=sys.stderr)
+        tries = [ranks, 1] if ranks > 1 else [1]
+        for nr in tries:
+            try:
+                _, _, err = O.run_reference(sp, U0, steps + 2, timing_warmup=2, ranks=nr)
+                for line in err.splitlines():
+                    if line.startswith("TIMING"):
+                        kv = dict(t.split("=") for t in line.split()[1:])
+                        return {"value": float(kv["cell_updates_per_s"]),
+                                "unit": "cell-updates/s", "cores": nr, "kind": "reference",
+                                "sample": sample + f"; reference CPU build (use_cuda=0), {nr} MPI "
+                                                   f"rank(s), one per core, z split over ranks",
+                                "host_cpus": os.cpu_count()}
+            except Exception as e:  # reference binary or mpiexec unusable on this host
+                print(f"[bench] reference CPU build with {nr} rank(s) unavailable here: {e}",
+                      file=sys.stderr)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     st = O.FlatState(U0, dtype)
     O.flat_evolve(sp, st, 2, mask_lanes=16 if dtype == np.float32 else 8)
@@ -173,8 +193,12 @@ def sph_sweep_params(rng, nr: int, nth: int):
 SPH_SCALARS = {"sph_stage": 28, "sph_stage_first": 21, "sph_stage_last": 21, "sph_ko": 14}
 
 
-def run_sph_sweep(args):
-    import torch
+def _sph_batch(args, runs: int, retire_scale: float = 0.0):
+    """The batched sweep's runs (grids and R_LC drawn from the seed).  With
+    retire_scale > 0 each run stops after retire_scale x the steps of 5
+    rotation periods (5 * 2 pi R_LC / dt), so runs retire at different
+    times."""
+    import math
 
     from coffeegpu_amd import initial_data as ic
     from coffeegpu_amd.solver import SimData, SimEnvironment
@@ -182,9 +206,8 @@ def run_sph_sweep(args):
 
     rng = np.random.default_rng(args.seed)
     shapes = [(512, 256), (1024, 512), (2048, 1024)]
-    envs, datas, cells, rlcs = [], [], 0, []
-    t0 = time.perf_counter()
-    for r in range(args.runs):
+    envs, datas, cells, rlcs, nsteps = [], [], [], [], []
+    for r in range(runs):
         nr, nth = shapes[int(rng.integers(0, len(shapes)))]
         sp, rlc = sph_sweep_params(rng, nr, nth)
         env = SimEnvironment(sp, np.float32, "cuda:0")
@@ -194,27 +217,39 @@ def run_sph_sweep(args):
             arr.set(f[name])
         envs.append(env)
         datas.append(d)
-        cells += nr * nth
+        cells.append(nr * nth)
         rlcs.append(rlc)
-    b = SphBatch(envs, datas, nsteps=[0] * args.runs)
+        nsteps.append(max(1, int(math.ceil(retire_scale * 5 * 2 * math.pi * rlc / sp.dt)))
+                      if retire_scale > 0 else 0)
+    b = SphBatch(envs, datas, nsteps=nsteps)
+    return b, shapes, np.array(cells), np.array(rlcs), np.array(nsteps), datas
+
+
+def sph_sweep_result(args, runs: int, steps: int, warmup: int) -> dict:
+    """Batched 2D pulsar sweep: `runs` runs advanced by one graph-replayed
+    launch sequence per step (runs never retire during the timing)."""
+    import torch
+
+    t0 = time.perf_counter()
+    b, shapes, cells_r, rlcs, _, datas = _sph_batch(args, runs)
+    cells = int(cells_r.sum())
     if args.tile >= 0:
         b.set_tile(args.tile)
         best, tile_ms = args.tile, []
     else:
         best, tile_ms = b.autotune(trials=3)
     b.use_graph(True)
-    print(f"[bench] sph-sweep {args.runs} runs, {cells / 1e6:.1f} M cells, set-up "
-          f"{time.perf_counter() - t0:.1f}s, tile {best} {tile_ms}", file=sys.stderr, flush=True)
-    b.step(args.warmup)
+    print(f"[bench] sph-sweep {runs} runs, {cells / 1e6:.1f} M cells, set-up "
+          f"{time.perf_counter() - t0:.1f}s, tile {best}", file=sys.stderr, flush=True)
+    b.step(warmup)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
-    b.step(args.steps)
+    b.step(steps)
     stop.record()
     torch.cuda.synchronize()
-    ms_total = start.elapsed_time(stop)
-    ms_per_step = ms_total / args.steps
+    ms_per_step = start.elapsed_time(stop) / steps
     value = cells / (ms_per_step * 1e-3)
     # per-kernel durations: direct launches with events (graphs are off while timing)
     b.set_timing(True)
@@ -225,31 +260,86 @@ def run_sph_sweep(args):
     cnt, ms = timing["sph_stage"]
     bytes_per_launch = SPH_SCALARS["sph_stage"] * 4 * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
-    lum = b.luminosity(1.5 * np.asarray(rlcs))
+    lum = b.luminosity(1.5 * rlcs)
+    res = b.residuals()
     # HBM bytes per middle-stage launch, measured for one tile and batch size
     tr = load_traffic("sph_sweep_fp32_traffic.json") or {}
     traffic = tr.get("bytes_per_launch") if (tr.get("tile") == best and
-                                             tr.get("runs") == args.runs and
+                                             tr.get("runs") == runs and
                                              tr.get("seed") == args.seed) else None
-    out = {
-        "metric": "cell-updates/s (batched 2D log-spherical pulsar sweep, 1 GPU)",
-        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (rotating dipoles, seeded grids and R_LC)",
-        "config": {"workload": f"sph-sweep {args.runs} runs, grids from {shapes}, R_LC~U[3,20], "
+    b.close()
+    del b, datas
+    return {
+        "value": value, "unit": "cell-updates/s", "ms_per_step": ms_per_step, "steps": steps,
+        "warmup": warmup,
+        "config": {"workload": f"sph-sweep {runs} runs, grids from {shapes}, R_LC~U[3,20], "
                                "r_out 60, absorbing r>48, hipGraph replay, autotuned tile",
-                   "cells": cells, "tile": best, "parallelism": "single GPU"},
+                   "cells": cells, "tile": best, "tile_ms": tile_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "sph_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "step_algorithmic_GBps": SCALARS_PER_CELL_UPDATE * 4 * value / 1e9,
+        "frac_step_616B": 616.0 * value / (HBM_PEAK_GBS * 1e9),
         "kernels": _kernel_table(timing, SPH_SCALARS, 4, cells),
         "luminosity_1p5RLC_median": float(np.median(lum)),
-        "cpu_baseline": None,
+        "residuals_max_divB": max(r["max_divB"] for r in res),
+        "residuals_nonfinite": sum(r["nonfinite"] for r in res),
     }
-    print(json.dumps(out))
+
+
+def sph_retiring_result(args, runs: int, scale: float) -> dict:
+    """The sweep run to its (scaled) end: run r stops after scale x the steps
+    of 5 rotation periods, so the batch thins out as short runs retire.
+    value = sum of the runs' cell-updates / wall time until the last run
+    retires (graph replay, inactive runs' work items skipped)."""
+    import torch
+
+    t0 = time.perf_counter()
+    b, _, cells_r, rlcs, nsteps, datas = _sph_batch(args, runs, retire_scale=scale)
+    if args.tile >= 0:
+        b.set_tile(args.tile)
+    else:
+        b.autotune(trials=3)
+    b.use_graph(True)
+    total = int((cells_r * nsteps).sum())
+    nmax = int(nsteps.max())
+    print(f"[bench] sph-sweep retiring: {runs} runs, {nsteps.min()}..{nmax} steps, set-up "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    done = 0
+    while done < nmax:  # chunks, so a long run prints progress
+        k = min(200, nmax - done)
+        b.step(k)
+        done += k
+        torch.cuda.synchronize()
+        print(f"[bench] sph-sweep retiring: {done}/{nmax} steps", file=sys.stderr, flush=True)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    act, _, st = b.status()
+    ok = (sum(act) == 0) and all(int(a) == int(e) for a, e in zip(st, nsteps))
     b.close()
+    del b, datas
+    return {"value": total / (ms * 1e-3), "unit": "cell-updates/s", "wall_ms": ms,
+            "cell_updates": total, "steps_min": int(nsteps.min()), "steps_max": nmax,
+            "retire_scale": scale, "all_runs_retired_at_target": bool(ok),
+            "config": f"{runs} runs, each to {scale} x the steps of 5 rotation periods"}
+
+
+def run_sph_sweep(args):
+    r = sph_sweep_result(args, args.runs, args.steps, args.warmup)
+    out = {"metric": "cell-updates/s (batched 2D log-spherical pulsar sweep, 1 GPU)",
+           "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (rotating dipoles, seeded grids and R_LC)",
+           "config": dict(r["config"], parallelism="single GPU"), "roofline": r["roofline"],
+           "cpu_baseline": None}
+    out.update({k: v for k, v in r.items() if k not in out and k not in ("config",)})
+    print(json.dumps(out))
 
 
 GR_SCALARS = {"gr_stage": 28, "gr_stage_first": 21, "gr_stage_last": 21, "gr_ko": 14,
@@ -331,16 +421,15 @@ def gr_wald_params(n: int, spin: float):
                      check_egb=True, problem=0, a=spin)
 
 
-def run_gr(args):
+def gr_result(args, n: int, spin: float, dtype, steps: int, warmup: int,
+              variant: int = 0, zchunk: int = 0) -> dict:
     import torch
 
     from coffeegpu_amd import initial_data as ic
     from coffeegpu_amd.gr import FieldSolverGREZ
     from coffeegpu_amd.solver import SimData, SimEnvironment
 
-    n = args.n if args.n != 512 else 384
-    dtype = np.float32 if args.dtype == "fp32" else np.float64
-    sp = gr_wald_params(n, args.spin)
+    sp = gr_wald_params(n, spin)
     env = SimEnvironment(sp, dtype, "cuda:0")
     t0 = time.perf_counter()
     f = ic.wald_uniform(env.grid(), sp.a, noise=1e-3, seed=args.seed, dtype=dtype)
@@ -349,10 +438,10 @@ def run_gr(args):
         arr.set(f[name])
     del f
     solver = FieldSolverGREZ(data, env)
-    solver.set_schedule(args.variant, args.zchunk)
-    print(f"[bench] gr-wald {n}^3 a={sp.a} variant {args.variant} set-up "
+    solver.set_schedule(variant, zchunk)
+    print(f"[bench] gr-wald {n}^3 a={sp.a} variant {variant} set-up "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         solver.evolve_fields(0.0)
     torch.cuda.synchronize()
     solver.set_timing(True)
@@ -360,44 +449,97 @@ def run_gr(args):
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         solver.evolve_fields(0.0)
     stop.record()
     torch.cuda.synchronize()
-    ms_per_step = start.elapsed_time(stop) / args.steps
+    ms_per_step = start.elapsed_time(stop) / steps
     timing = solver.get_timing()
+    solver.set_timing(False)
     cells = n ** 3
     value = cells / (ms_per_step * 1e-3)
     esz = np.dtype(dtype).itemsize
     cnt, ms = timing["gr_stage"]
     bytes_per_launch = GR_SCALARS["gr_stage"] * esz * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
-    W = solver.total_energy(data.B)
-    traffic = load_traffic(f"gr_stage_{n}_{args.dtype}_traffic.json")
-    if traffic and traffic.get("variant", 0) != args.variant:
-        traffic = None  # measured for another schedule
-    out = {
-        "metric": "cell-updates/s (Kerr-Schild GR Wald problem, 1 GPU)",
-        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if dtype == np.float32 else "f64",
-        "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
+    res = solver.residuals()
+    dname = "fp32" if dtype == np.float32 else "fp64"
+    traffic = load_traffic(f"gr_stage_{n}_{dname}_traffic.json")
+    if traffic and (traffic.get("variant", 0) != variant or traffic.get("spin", spin) != spin):
+        traffic = None  # measured for another schedule or spin
+    solver.close()
+    del solver, data
+    torch.cuda.empty_cache()
+    return {
+        "value": value, "unit": "cell-updates/s", "ms_per_step": ms_per_step, "steps": steps,
+        "warmup": warmup, "dtype": "f32" if dtype == np.float32 else "f64",
         "config": {"workload": f"gr-wald {n}^3 box [-20,20]^3 a={sp.a}, pml 32 (sigpml 10, "
                                "pmllen 1), horizon excision, FP64 metric on the fly (halo "
                                "points of stages 2-5 from a precomputed table)",
-                   "cells": cells, "schedule": "tiled" if args.variant != 1 else "point+stage",
-                   "parallelism": "single GPU"},
+                   "cells": cells, "schedule": "tiled" if variant != 1 else "point+stage"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "gr_stage",
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel": "gr_stage", "algorithmic_bytes_per_launch": bytes_per_launch},
         "kernels": _kernel_table(timing, GR_SCALARS, esz, cells),
-        "energy_B_after": W,
-        "cpu_baseline": None,
+        "residuals_after": res,
     }
+
+
+def run_gr(args):
+    n = args.n if args.n != 512 else 384
+    dtype = np.float32 if args.dtype == "fp32" else np.float64
+    r = gr_result(args, n, args.spin, dtype, args.steps, args.warmup, args.variant, args.zchunk)
+    out = {"metric": "cell-updates/s (Kerr-Schild GR Wald problem, 1 GPU)",
+           "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": r["dtype"],
+           "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
+           "config": dict(r["config"], parallelism="single GPU"), "roofline": r["roofline"],
+           "kernels": r["kernels"], "residuals_after": r["residuals_after"],
+           "cpu_baseline": None}
     print(json.dumps(out))
-    solver.close()
+
+
+def attainable_bandwidth(total_mib: int = 8192) -> dict:
+    """HBM bandwidth of a plain read + write stream (coffee_stream_copy,
+    16 bytes per lane), best over loads in flight per lane and grid sizes;
+    bytes counted = bytes read + bytes written."""
+    import ctypes
+
+    import torch
+
+    from coffeegpu_amd import _lib
+
+    L = _lib.load()
+    n = total_mib * (1 << 20) // 2
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    best = {"GBps": 0.0}
+    rows = []
+    for unroll in (1, 2, 4, 8):
+        for blocks in (1024, 2048, 4096, 8192):
+            call = lambda: L.coffee_stream_copy(ctypes.c_void_p(src.data_ptr()),  # noqa: E731
+                                                ctypes.c_void_p(dst.data_ptr()), n, unroll,
+                                                blocks, s)
+            for _ in range(2):
+                _lib.check(call())
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(5):
+                call()
+            e1.record()
+            torch.cuda.synchronize()
+            gbs = 2.0 * n * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+            rows.append([unroll, blocks, round(gbs, 1)])
+            if gbs > best["GBps"]:
+                best = {"GBps": gbs, "unroll": unroll, "blocks": blocks}
+    del src, dst
+    torch.cuda.empty_cache()
+    best.update({"bytes_per_copy_read": n, "kernel": "coffee_stream_copy (16 B per lane)",
+                 "sweep_unroll_blocks_GBps": rows})
+    return best
 
 
 def main():
@@ -462,6 +604,10 @@ def main():
     cells = n ** 3
     value = cells / (ms_per_step * 1e-3)
 
+    solver.close()
+    del solver, data
+    torch.cuda.empty_cache()
+
     esz = np.dtype(dtype).itemsize
     cnt, ms = timing["ez_stage"]
     avg_ms = ms / cnt
@@ -499,11 +645,24 @@ def main():
                      "kernel": "ez_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "step_algorithmic_GBps": step_gbs,
+        # whole-step fraction of peak with the two byte counts in use: 560 B
+        # (dead transfers of stages 1 and 5 dropped) and 616 B (5 x 28 + 14 scalars)
+        "frac_step_560B": SCALARS_PER_CELL_UPDATE * esz * value / (HBM_PEAK_GBS * 1e9),
+        "frac_step_616B": 154 * esz * value / (HBM_PEAK_GBS * 1e9),
         "kernels": kernels,
         "wall_s_timed": wall,
         "residuals_after": res,
         "cpu_baseline": cpu,
     }
+    if not args.no_extras:
+        att = attainable_bandwidth()
+        out["attainable_GBps"] = att["GBps"]
+        out["attainable"] = att
+        out["frac_of_attainable"] = achieved / att["GBps"]
+        print(f"[bench] attainable {att['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+        out["sph_sweep"] = sph_sweep_result(args, 256, args.extra_steps, 3)
+        out["sph_sweep_retiring"] = sph_retiring_result(args, 256, args.retire_scale)
+        out["gr_wald"] = gr_result(args, 384, 0.9, np.float32, args.extra_steps, 2)
     print(json.dumps(out))
 
 

@@ -55,7 +55,10 @@ class CoffeeParams(ctypes.Structure):
 class CoffeeFields(ctypes.Structure):
     _fields_ = [("E", _P3), ("B", _P3), ("P", ctypes.c_void_p), ("dE", _P3), ("dB", _P3),
                 ("dP", ctypes.c_void_p), ("Etmp", _P3), ("Btmp", _P3), ("Ptmp", ctypes.c_void_p),
-                ("Bbg", _P3), ("J", _P3), ("divE", ctypes.c_void_p), ("divB", ctypes.c_void_p)]
+                ("Bbg", _P3), ("J", _P3), ("divE", ctypes.c_void_p), ("divB", ctypes.c_void_p),
+                ("dU_KO", ctypes.c_void_p), ("dU_Epar", ctypes.c_void_p),
+                ("dU_EgtB", ctypes.c_void_p), ("dU_KO_cum", ctypes.c_void_p),
+                ("dU_Epar_cum", ctypes.c_void_p), ("dU_EgtB_cum", ctypes.c_void_p)]
 
 
 _V, _I, _L = ctypes.c_void_p, ctypes.c_int, ctypes.c_long
@@ -89,6 +92,8 @@ SIGNATURES = {
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "coffee_sph_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "coffee_sph_batch_residuals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "coffee_sph_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "coffee_sph_num_tiles": (ctypes.c_int, []),
     "coffee_sph_tile_shape": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -124,6 +129,8 @@ SIGNATURES = {
     "coffee_unet_corotate": (ctypes.c_int, [_V * 6, _I, _I, _I, _I, _I, _I, _D, _D, _D, _D, _V]),
     "coffee_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_void_p]),
+    "coffee_stream_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,
                                                  ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_void_p]),

@@ -37,11 +37,7 @@ def hipcc() -> str:
 
 # (name, source, extra flags): C++ drivers over the host surface
 DRIVERS = [("coffee_drive_f32", "drivers/coffee_drive.cpp", []),
-           ("coffee_drive_f64", "drivers/coffee_drive.cpp", ["-DUSE_DOUBLE"]),
-           ("coffee_drive_sph_f32", "drivers/coffee_drive_sph.cpp", []),
-           ("coffee_drive_sph_f64", "drivers/coffee_drive_sph.cpp", ["-DUSE_DOUBLE"]),
-           ("coffee_drive_gr_f32", "drivers/coffee_drive_gr.cpp", []),
-           ("coffee_drive_gr_f64", "drivers/coffee_drive_gr.cpp", ["-DUSE_DOUBLE"])]
+           ("coffee_drive_f64", "drivers/coffee_drive.cpp", ["-DUSE_DOUBLE"])]
 
 
 def _run(cmd, what: str, verbose: bool):

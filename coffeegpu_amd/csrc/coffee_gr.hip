@@ -267,7 +267,35 @@ int gr_energy_impl(coffee_ctx* c, void* const comp[3], double* out, hipStream_t 
   return COFFEE_OK;
 }
 
+template <typename T>
+int gr_residuals_impl(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s) {
+  GrState* S = gr_state(c);
+  const GrK<T>& k = grk<T>(S);
+  FlatIO<T> io{};
+  for (int q = 0; q < 3; q++) {
+    if (!f->E[q] || !f->B[q]) return fail(c, COFFEE_ERR_INVALID, "D/B pointers must be set");
+    io.in[q] = (const T*)f->E[q];
+    io.in[3 + q] = (const T*)f->B[q];
+  }
+  if (!f->P) return fail(c, COFFEE_ERR_INVALID, "P pointer must be set");
+  io.in[6] = (const T*)f->P;
+  int rc = timed(c, "residuals", s, [&] {
+    gr_residual_kernel<T><<<c->part_blocks, 256, 0, s>>>(k, io, c->d_part);
+  });
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * NRED * c->part_blocks,
+                            hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  fold_residuals(c->h_part.data(), c->part_blocks, out);
+  return COFFEE_OK;
+}
+
 }  // namespace
+
+int gr_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s) {
+  return c->scalar == 4 ? gr_residuals_impl<float>(c, f, out, s)
+                        : gr_residuals_impl<double>(c, f, out, s);
+}
 
 int gr_create(coffee_ctx* c) {
   GrState* S = new GrState();

@@ -133,6 +133,24 @@ __global__ void fill_kernel(T* p, unsigned long long n, T v) {
     p[i] = v;
 }
 
+// Streaming copy of n 16-byte words: each thread issues U loads (grid-strided,
+// so a wavefront's loads are contiguous 1 KiB rows) before its U stores.
+template <int U>
+__global__ void __launch_bounds__(256)
+    stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                       unsigned long long n) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n; i += U * stride) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++) dst[i + u * stride] = v[u];
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 template <typename T>
 int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bool wrap,
                 int nfields = 7) {
@@ -273,7 +291,12 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
   }
   const T t1 = t0 + dt;
   return timed(c, "ez_ko", s, [&] {
-    if (tiled) {
+    if (dg) {
+      // diagnostics mode: the one-thread-per-cell KO kernel with the energy
+      // accounting (bitwise the same field results as the tiled one)
+      if (star) ez_ko_kernel<T, true, true><<<grid, block, 0, s>>>(k, io, t1);
+      else ez_ko_kernel<T, false, true><<<grid, block, 0, s>>>(k, io, t1);
+    } else if (tiled) {
       if (star) ez_ko_tiled<T, TX, TY, true><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
       else ez_ko_tiled<T, TX, TY, false><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
     } else {
@@ -327,6 +350,21 @@ int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t
   io.Un[6] = (T*)f->Ptmp;
   io.divE = (T*)f->divE;
   io.divB = (T*)f->divB;
+  if (c->diagnostics) {
+    void* const acc[6] = {f->dU_KO, f->dU_Epar, f->dU_EgtB, f->dU_KO_cum, f->dU_Epar_cum,
+                          f->dU_EgtB_cum};
+    for (int q = 0; q < 6; q++) io.acc[q] = (T*)acc[q];
+    // the per-step arrays start every step at zero (field_solver_EZ.cpp:1020-1022)
+    const unsigned long long n = cells(c->grid);
+    for (int q = 0; q < 3; q++)
+      if (io.acc[q]) {
+        rc = timed(c, "fill", s, [&] {
+          fill_kernel<T><<<(int)std::min<unsigned long long>((n + 255) / 256, 8192), 256, 0, s>>>(
+              io.acc[q], n, T(0));
+        });
+        if (rc) return rc;
+      }
+  }
 
   const bool star = k.problem == 1 || k.problem == 2;
   int tx = 64, ty = 8;
@@ -359,16 +397,7 @@ int residuals_impl(coffee_ctx* c, const coffee_fields* f, double out[5], hipStre
   HIP_TRY(c, hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * NRED * c->part_blocks,
                             hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
-  double r[5] = {0, 0, 0, 0, 0};
-  for (int b = 0; b < c->part_blocks; b++) {
-    const double* p = &c->h_part[b * NRED];
-    r[0] += p[0];
-    r[1] = std::max(r[1], p[1]);
-    r[2] = std::max(r[2], p[2]);
-    r[3] = std::max(r[3], p[3]);
-    r[4] += p[4];
-  }
-  for (int q = 0; q < 5; q++) out[q] = r[q];
+  fold_residuals(c->h_part.data(), c->part_blocks, out);
   return COFFEE_OK;
 }
 
@@ -522,9 +551,13 @@ int coffee_total_energy(coffee_ctx* c, void* const comp[3], double* out, void* s
 
 int coffee_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], void* stream) {
   if (!c || !f || !out) return fail(c, COFFEE_ERR_INVALID, "null argument");
-  if (c->solver != COFFEE_SOLVER_EZ)
-    return fail(c, COFFEE_ERR_STATE, "residuals are available for the flat solver");
   hipStream_t s = (hipStream_t)stream;
+  if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) {
+    if (c->is_batch)
+      return fail(c, COFFEE_ERR_STATE, "batches report residuals per run: coffee_sph_batch_residuals");
+    return coffee::sph_residuals(c, f, out, s);
+  }
+  if (c->solver == COFFEE_SOLVER_GR_EZ) return coffee::gr_residuals(c, f, out, s);
   return c->scalar == 4 ? residuals_impl<float>(c, f, out, s) : residuals_impl<double>(c, f, out, s);
 }
 
@@ -547,6 +580,28 @@ int coffee_fill(void* p, unsigned long long n, int scalar_bytes, double value, v
     fill_kernel<double><<<blocks, 256, 0, s>>>((double*)p, n, value);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, std::string("fill: ") + hipGetErrorString(e));
+  return COFFEE_OK;
+}
+
+int coffee_stream_copy(const void* src, void* dst, unsigned long long bytes, int unroll,
+                       int blocks, void* stream) {
+  if (!src || !dst) return fail(nullptr, COFFEE_ERR_INVALID, "null pointer");
+  if (bytes % 16 || ((unsigned long long)src | (unsigned long long)dst) % 16)
+    return fail(nullptr, COFFEE_ERR_INVALID, "bytes and pointers must be multiples of 16");
+  if (blocks < 1 || blocks > 65536) return fail(nullptr, COFFEE_ERR_INVALID, "blocks out of range");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned long long n = bytes / 16;
+  const float4* a = (const float4*)src;
+  float4* b = (float4*)dst;
+  switch (unroll) {
+    case 1: stream_copy_kernel<1><<<blocks, 256, 0, s>>>(a, b, n); break;
+    case 2: stream_copy_kernel<2><<<blocks, 256, 0, s>>>(a, b, n); break;
+    case 4: stream_copy_kernel<4><<<blocks, 256, 0, s>>>(a, b, n); break;
+    case 8: stream_copy_kernel<8><<<blocks, 256, 0, s>>>(a, b, n); break;
+    default: return fail(nullptr, COFFEE_ERR_INVALID, "unroll must be 1, 2, 4 or 8");
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, std::string("stream_copy: ") + hipGetErrorString(e));
   return COFFEE_OK;
 }
 

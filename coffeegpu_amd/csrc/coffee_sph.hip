@@ -526,6 +526,47 @@ int sph_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s) {
                         : sph_energy_impl<double>(c, comp, out, s);
 }
 
+// Residual rows of every run: c->part_blocks workgroups, part_blocks / nruns
+// per run, folded per run into out[run * 5 ...].
+template <typename T>
+int sph_residuals_runs(coffee_ctx* c, double* out, hipStream_t s) {
+  SphState<T>* S = sph_state<T>(c);
+  const int nr = (int)S->h_runs.size();
+  const int bpr = c->part_blocks / nr;
+  if (bpr < 1 || !c->d_part) return fail(c, COFFEE_ERR_STATE, "no residual buffer");
+  int rc = timed(c, "sph_residuals", s, [&] {
+    sph_residual_kernel<T><<<bpr * nr, 256, 0, s>>>(S->d_runs, bpr, c->d_part);
+  });
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(c->h_part.data(), c->d_part, sizeof(double) * NRED * bpr * nr,
+                            hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  for (int r = 0; r < nr; r++) fold_residuals(c->h_part.data() + (size_t)r * bpr * NRED, bpr, out + 5 * r);
+  return COFFEE_OK;
+}
+
+// Single run: the state arrays of this call.
+template <typename T>
+int sph_residuals_impl(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s) {
+  SphState<T>* S = sph_state<T>(c);
+  SphRun<T>& R = S->h_runs[0];
+  for (int q = 0; q < 3; q++) {
+    if (!f->E[q] || !f->B[q]) return fail(c, COFFEE_ERR_INVALID, "E, B must be set");
+    R.U[0][q] = (T*)f->E[q];
+    R.U[0][3 + q] = (T*)f->B[q];
+  }
+  if (!f->P) return fail(c, COFFEE_ERR_INVALID, "P must be set");
+  R.U[0][6] = (T*)f->P;
+  HIP_TRY(c, hipMemcpyAsync(S->d_runs, &R, sizeof(SphRun<T>), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return sph_residuals_runs<T>(c, out, s);
+}
+
+int sph_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s) {
+  return c->scalar == 4 ? sph_residuals_impl<float>(c, f, out, s)
+                        : sph_residuals_impl<double>(c, f, out, s);
+}
+
 }  // namespace coffee
 
 using namespace coffee;
@@ -604,6 +645,14 @@ int coffee_sph_batch_create(int scalar_bytes, int nruns, const coffee_grid* grid
   (void)hipGetDevice(&c->device);
   int rc = scalar_bytes == 4 ? batch_create<float>(c, nruns, grids, params, fields, nsteps, t0)
                              : batch_create<double>(c, nruns, grids, params, fields, nsteps, t0);
+  if (rc == COFFEE_OK) {
+    // residual partial rows: about 1024 workgroups over the batch, at least
+    // one per run
+    c->part_blocks = std::max(1, 1024 / nruns) * nruns;
+    c->h_part.assign((size_t)NRED * c->part_blocks, 0.0);
+    if (hipMalloc(&c->d_part, sizeof(double) * NRED * c->part_blocks) != hipSuccess)
+      rc = fail(c, COFFEE_ERR_HIP, "hipMalloc residual buffer failed");
+  }
   if (rc) {
     std::string m = c->err;
     coffee_ctx_destroy(c);
@@ -672,6 +721,13 @@ int coffee_sph_batch_luminosity(coffee_batch* c, const double* radius, double* o
   hipStream_t s = (hipStream_t)stream;
   return c->scalar == 4 ? luminosity_impl<float>(c, radius, out, s)
                         : luminosity_impl<double>(c, radius, out, s);
+}
+
+int coffee_sph_batch_residuals(coffee_batch* c, double* out, void* stream) {
+  if (!is_batch(c)) return fail(c, COFFEE_ERR_STATE, "not a spherical batch");
+  if (!out) return fail(c, COFFEE_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return c->scalar == 4 ? sph_residuals_runs<float>(c, out, s) : sph_residuals_runs<double>(c, out, s);
 }
 
 int coffee_sph_batch_status(coffee_batch* c, int* active, double* t, int* step) {

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -124,5 +125,22 @@ int gr_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s);
 int sph_create(coffee_ctx* c);
 int sph_step(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t s);
 int sph_energy(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s);
+int sph_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s);
+int gr_residuals(coffee_ctx* c, const coffee_fields* f, double out[5], hipStream_t s);
+
+// Folds nblocks partial rows of NRED doubles (sum, max, max, max, sum) in
+// workgroup order.
+inline void fold_residuals(const double* part, int nblocks, double out[5]) {
+  double r[5] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < nblocks; b++) {
+    const double* p = part + b * 5;
+    r[0] += p[0];
+    r[1] = std::max(r[1], p[1]);
+    r[2] = std::max(r[2], p[2]);
+    r[3] = std::max(r[3], p[3]);
+    r[4] += p[4];
+  }
+  for (int q = 0; q < 5; q++) out[q] = r[q];
+}
 
 }  // namespace coffee

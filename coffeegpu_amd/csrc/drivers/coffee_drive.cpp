@@ -1,19 +1,23 @@
-// Flat EZ driver over the C++ host surface, shaped like the reference's
-// main loops (src/main-wave.cpp:20-90, src/main-alfven3d.cpp):
+// Driver over the C++ host surface, shaped like the reference's main loops
+// (src/main-wave.cpp:20-90, src/main-alfven3d.cpp, src/main-wald.cpp):
 //   sim_environment env(&argc, &argv);   // reads ./config.toml
 //   sim_data data(env);
 //   <initial data>                       // here: read from a CFLD file
-//   field_solver_EZ solver(data, env);
+//   <solver> solver(data, env);          // field_solver_EZ / _EZ_spherical / _gr_EZ
 //   for (step...) { solver.evolve_fields(time); time += dt; }
 //
-// usage: coffee_drive <in.cfld> <out.cfld> <nsteps>   (run where config.toml is)
+// usage: coffee_drive <in.cfld> <out.cfld> <nsteps> [ez|sph|gr]   (run where config.toml is)
 // CFLD: char[4] "CFLD", int32 {1, scalar_bytes, dims0, dims1, dims2, 7}, then
-// Ex,Ey,Ez,Bx,By,Bz,P in Fortran order with guards.
+// Ex,Ey,Ez,Bx,By,Bz,P in Fortran order with guards (Dx,Dy,Dz for gr).
+// Prints the energies of B and E and the residual monitor of the final state.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "coffee/field_solver_EZ.h"
+#include "coffee/field_solver_EZ_spherical.h"
+#include "coffee/field_solver_gr_EZ.h"
 
 using namespace Coffee;
 
@@ -23,9 +27,31 @@ static multi_array<Scalar>& field(sim_data& d, int c) {
   return d.P;
 }
 
+template <typename Solver>
+static void run(sim_data& data, sim_environment& env, int nsteps) {
+  Solver solver(data, env);
+  Scalar time = 0.0;
+  for (int step = 0; step < nsteps; step++) {
+    solver.evolve_fields(time);
+    time += env.params().dt;
+  }
+  Scalar wB = solver.total_energy(data.B);
+  Scalar wE = solver.total_energy(data.E);
+  double res[5];
+  solver.residuals(res);
+  printf("energy B %.17g E %.17g\n", (double)wB, (double)wE);
+  printf("residuals energy %.17g max_divB %.17g max_EdotB %.17g max_E2mB2 %.17g nonfinite %.0f\n",
+         res[0], res[1], res[2], res[3], res[4]);
+}
+
 int main(int argc, char* argv[]) {
   if (argc < 4) {
-    fprintf(stderr, "usage: %s in.cfld out.cfld nsteps\n", argv[0]);
+    fprintf(stderr, "usage: %s in.cfld out.cfld nsteps [ez|sph|gr]\n", argv[0]);
+    return 2;
+  }
+  const std::string kind = argc > 4 ? argv[4] : "ez";
+  if (kind != "ez" && kind != "sph" && kind != "gr") {
+    fprintf(stderr, "unknown solver %s\n", kind.c_str());
     return 2;
   }
   const char* in = argv[1];
@@ -57,15 +83,14 @@ int main(int argc, char* argv[]) {
   }
   fclose(f);
 
-  field_solver_EZ solver(data, env);
-  Scalar time = 0.0;
-  for (int step = 0; step < nsteps; step++) {
-    solver.evolve_fields(time);
-    time += env.params().dt;
+  try {
+    if (kind == "sph") run<field_solver_EZ_spherical>(data, env, nsteps);
+    else if (kind == "gr") run<field_solver_gr_EZ>(data, env, nsteps);
+    else run<field_solver_EZ>(data, env, nsteps);
+  } catch (const std::exception& e) {
+    fprintf(stderr, "%s\n", e.what());
+    return 5;
   }
-  Scalar wB = solver.total_energy(data.B);
-  Scalar wE = solver.total_energy(data.E);
-  printf("energy B %.17g E %.17g\n", (double)wB, (double)wE);
 
   f = fopen(out, "wb");
   if (!f) return 4;

@@ -61,6 +61,9 @@ struct FlatIO {
   T* __restrict__ J[3];
   T* __restrict__ divE;
   T* __restrict__ divB;
+  // energy accounting of diagnostics mode (any may be null): dU_KO,
+  // dU_Epar, dU_EgtB, then their _cum arrays
+  T* __restrict__ acc[6];
 };
 
 __device__ __forceinline__ int wrap_idx(int c, int g, int n, int per) {
@@ -302,6 +305,36 @@ __device__ __forceinline__ void project_cell(T& ex, T& ey, T& ez, T bx, T by, T 
   }
 }
 
+// project_cell with the energy accounting of clean_epar_check_eGTb
+// (field_solver_EZ.cpp:588-623): dU_Epar += |E'|^2 - |E|^2 after the E.B
+// removal, dU_EgtB += |E''|^2 - |E|^2 after the E>B cap (both from the
+// pre-projection |E|^2), and the same into the _cum arrays.
+template <typename T>
+__device__ __forceinline__ void project_cell_account(const FlatIO<T>& io, int c, T& ex, T& ey,
+                                                     T& ez, T bx, T by, T bz) {
+  const T u0 = ex * ex + ey * ey + ez * ez;
+  T B2 = bx * bx + by * by + bz * bz;
+  B2 = B2 >= TinyOf<T>::v ? B2 : TinyOf<T>::v;
+  const T EB = bx * ex + by * ey + bz * ez;
+  const T f = EB / B2;
+  ex -= f * bx;
+  ey -= f * by;
+  ez -= f * bz;
+  const T E2 = ex * ex + ey * ey + ez * ez;
+  const T u1 = E2;
+  if (E2 > B2) {
+    const T s = sqrt(B2 / E2);
+    ex *= s;
+    ey *= s;
+    ez *= s;
+  }
+  const T u2 = ex * ex + ey * ey + ez * ez;
+  if (io.acc[1]) io.acc[1][c] += u1 - u0;
+  if (io.acc[4]) io.acc[4][c] += u1 - u0;
+  if (io.acc[2]) io.acc[2][c] += u2 - u0;
+  if (io.acc[5]) io.acc[5][c] += u2 - u0;
+}
+
 template <typename T>
 __device__ __forceinline__ T pml_sigma(T x, T xl, T xh, T scale, T sig0) {
   // damping_boundary.h:14-22
@@ -412,7 +445,8 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
 #pragma unroll
   for (int q = 0; q < 7; q++) u[q] = fma(du[q], Bs, u[q]);
 
-  project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
+  if (DIAG) project_cell_account(io, c, u[0], u[1], u[2], u[3], u[4], u[5]);
+  else project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
   if (STAR) {
     const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
     star_boundary(k, x, y, z, tstage, u[0], u[1], u[2], u[3], u[4], u[5]);
@@ -446,10 +480,21 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
 // Per-cell end of the Kreiss-Oliger pass (field_solver_EZ.cpp:390-457 with
 // the projection and boundary of .cpp:1057-1067): u[0..5] already hold
 // f + (KOeps/64) sum_axes delta^6 f, u[6] = P (untouched on the CPU path).
-template <typename T, bool STAR>
+// DIAG: e_before = |E|^2 + |B|^2 before the KO update; dU_KO = the change
+// (field_solver_EZ.cpp:447-453), then the projection's accounting.
+template <typename T, bool STAR, bool DIAG = false>
 __device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T>& io, int i,
-                                               int j, int kk, int c, T (&u)[7], T t) {
-  project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
+                                               int j, int kk, int c, T (&u)[7], T t,
+                                               T e_before = T(0)) {
+  if (DIAG) {
+    const T e_after = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) +
+                      (u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);
+    if (io.acc[0]) io.acc[0][c] = e_after - e_before;
+    if (io.acc[3]) io.acc[3][c] += e_after - e_before;
+    project_cell_account(io, c, u[0], u[1], u[2], u[3], u[4], u[5]);
+  } else {
+    project_cell(u[0], u[1], u[2], u[3], u[4], u[5]);
+  }
   if (STAR) {
     const T x = node_pos(k, 0, i), y = node_pos(k, 1, j), z = node_pos(k, 2, kk);
     star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
@@ -493,8 +538,9 @@ __global__ void __launch_bounds__(256)
   stage_cell<T, FIRST, LAST, DIAG, STAR>(k, io, i, j, kk, c, d, u, dp, As, Bs, tstage);
 }
 
-// Simple Kreiss_Oliger kernel (one thread per cell).
-template <typename T, bool STAR>
+// Simple Kreiss_Oliger kernel (one thread per cell); DIAG adds the energy
+// accounting (diagnostics mode always runs this kernel).
+template <typename T, bool STAR, bool DIAG = false>
 __global__ void __launch_bounds__(256) ez_ko_kernel(const FlatK<T> k, const FlatIO<T> io, const T t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + k.guard[0];
   const int j = blockIdx.y * blockDim.y + threadIdx.y + k.guard[1];
@@ -517,7 +563,14 @@ __global__ void __launch_bounds__(256) ez_ko_kernel(const FlatK<T> k, const Flat
     u[q] = fma(ko, k.KOfac, f[c]);
   }
   u[6] = io.in[6][c];
-  ko_cell_finish<T, STAR>(k, io, i, j, kk, c, u, t);
+  T e_before = T(0);
+  if (DIAG) {
+    T v[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) v[q] = io.in[q][c];
+    e_before = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) + (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+  }
+  ko_cell_finish<T, STAR, DIAG>(k, io, i, j, kk, c, u, t, e_before);
 }
 
 // Guard slabs of one axis: periodic -> copy from the interior one period

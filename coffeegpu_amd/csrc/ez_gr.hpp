@@ -472,4 +472,74 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Residual monitor over interior cells (no reference counterpart; the
+// reference stores per-cell divB/divD each stage, field_solver_gr_EZ.cu:336-338),
+// NRED doubles per workgroup, in FP64 from the Scalar arrays:
+//   sum (D_i D^i + B_i B^i) sqrt(gamma),
+//   max |div B|, div B = d_i(sqrt(gamma) B^i) / sqrt(gamma) (div4, .cu:44-88),
+//   max |D_i B^i| / max(B_i B^i, TINY), max (D_i D^i - B_i B^i) / max(B_i B^i, TINY),
+//   number of cells with a non-finite D, B or P.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gr_residual_kernel(const GrK<T> k, const FlatIO<T> io, double* __restrict__ part) {
+  __shared__ double sh[NRED][256];
+  double acc[NRED] = {0, 0, 0, 0, 0};
+  const int total = k.n[0] * k.n[1] * k.n[2];
+  const double tiny = (double)TinyOf<T>::v;
+  const int o[4] = {-2, -1, 1, 2};
+  const double wt[4] = {1.0, -8.0, 8.0, -1.0};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int i = t % k.n[0] + k.guard[0];
+    const int rr = t / k.n[0];
+    const int j = rr % k.n[1] + k.guard[1];
+    const int kk = rr / k.n[1] + k.guard[2];
+    const int c = i + j * k.sy + kk * k.sz;
+    const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
+    const Cks m = cks_point(k.a, x, y, z);
+    double u[7];
+    for (int q = 0; q < 7; q++) u[q] = (double)io.in[q][c];
+    const double lD = m.lx * u[0] + m.ly * u[1] + m.lz * u[2];
+    const double lB = m.lx * u[3] + m.ly * u[4] + m.lz * u[5];
+    const double D2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + m.s * lD * lD;
+    const double B2 = u[3] * u[3] + u[4] * u[4] + u[5] * u[5] + m.s * lB * lB;
+    const double DB = u[0] * u[3] + u[1] * u[4] + u[2] * u[5] + m.s * lD * lB;
+    const double B2c = B2 > tiny ? B2 : tiny;
+    const double sg = sqrt(1.0 + m.s);
+    double div = 0;
+    const int st[3] = {1, k.sy, k.sz};
+    for (int ax = 0; ax < 3; ax++) {
+      double acc_ax = 0;
+      for (int q = 0; q < 4; q++) {
+        const double px = x + (ax == 0 ? o[q] * k.delta[0] : 0.0);
+        const double py = y + (ax == 1 ? o[q] * k.delta[1] : 0.0);
+        const double pz = z + (ax == 2 ? o[q] * k.delta[2] : 0.0);
+        const double sgt = sqrt(1.0 + cks_point(k.a, px, py, pz).s);
+        acc_ax += wt[q] * (double)io.in[3 + ax][c + o[q] * st[ax]] * sgt;
+      }
+      div += acc_ax / (12.0 * k.delta[ax]);
+    }
+    div /= sg;
+    const bool finite = isfinite(D2) && isfinite(B2) && isfinite(u[6]);
+    acc[0] += (D2 + B2) * sg;
+    acc[1] = fmax(acc[1], fabs(div));
+    acc[2] = fmax(acc[2], fabs(DB) / B2c);
+    acc[3] = fmax(acc[3], (D2 - B2) / B2c);
+    acc[4] += finite ? 0.0 : 1.0;
+  }
+  for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + w]);
+      sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + w]);
+      sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + w]);
+      sh[4][threadIdx.x] += sh[4][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NRED; q++) part[blockIdx.x * NRED + q] = sh[q][0];
+}
+
 }  // namespace coffee

@@ -851,4 +851,71 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * M_PI * red[0] * (double)R.dy;
 }
 
+// Residual monitor of every run of a batch over its interior cells (no
+// reference counterpart; the reference stores per-cell divB/divE each stage,
+// field_solver_EZ_spherical.cu:325-329).  Workgroup b of run `run` =
+// blockIdx.x / bpr writes NRED doubles to part[blockIdx.x * NRED]:
+//   sum (E_i E^i + B_i B^i) sqrt(gamma) dx dtheta,
+//   max |div B|, div B = d_i(sqrt(gamma) B^i) / sqrt(gamma) (div4_sph, .cu:99-124;
+//     cells on the axis, where sqrt(gamma) = 0, are skipped),
+//   max |E_i B^i| / max(B_i B^i, TINY), max (E_i E^i - B_i B^i) / max(B_i B^i, TINY),
+//   number of cells with a non-finite E, B or P.
+// Evaluated in FP64 from the Scalar arrays.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    sph_residual_kernel(const SphRun<T>* __restrict__ runs, int bpr, double* __restrict__ part) {
+  __shared__ double sh[NRED][256];
+  const int run = blockIdx.x / bpr, b = blockIdx.x % bpr;
+  const SphRun<T>& R = runs[run];
+  double acc[NRED] = {0, 0, 0, 0, 0};
+  const int total = R.n0 * R.n1;
+  const double tiny = (double)TinyOf<T>::v;
+  const T* const* U = R.U[0];
+  const int sy = R.dims0;
+  for (int t = b * (int)blockDim.x + (int)threadIdx.x; t < total; t += bpr * (int)blockDim.x) {
+    const int i = t % R.n0 + R.g0, j = t / R.n0 + R.g1;
+    const int c = i + j * sy;
+    const double r = (double)R.r[i], s = (double)R.sth[j];
+    const double g1 = r * r, g3 = (r * s) * (r * s);
+    double u[7];
+    for (int q = 0; q < 7; q++) u[q] = (double)gld(U[q] + c);
+    const double E2 = g1 * (u[0] * u[0] + u[1] * u[1]) + g3 * u[2] * u[2];
+    const double B2 = g1 * (u[3] * u[3] + u[4] * u[4]) + g3 * u[5] * u[5];
+    const double EB = g1 * (u[0] * u[3] + u[1] * u[4]) + g3 * u[2] * u[5];
+    const double B2c = B2 > tiny ? B2 : tiny;
+    const double sg = r * r * r * s;
+    bool finite = isfinite(E2) && isfinite(B2) && isfinite(u[6]);
+    acc[0] += (E2 + B2) * fabs(sg) * (double)R.dx * (double)R.dy;
+    if (fabs(sg) > tiny) {
+      double tx = 0, ty = 0;
+      const int o[4] = {-2, -1, 1, 2};
+      const double w[4] = {1.0, -8.0, 8.0, -1.0};
+      for (int q = 0; q < 4; q++) {
+        const double ri = (double)R.r[i + o[q]];
+        tx += w[q] * (double)gld(U[3] + c + o[q]) * ri * ri * ri * s;
+        ty += w[q] * (double)gld(U[4] + c + o[q] * sy) * r * r * r * (double)R.sth[j + o[q]];
+      }
+      const double div = (tx * (double)R.inv_dx + ty * (double)R.inv_dy) / 12.0 / sg;
+      acc[1] = fmax(acc[1], fabs(div));
+    }
+    acc[2] = fmax(acc[2], fabs(EB) / B2c);
+    acc[3] = fmax(acc[3], (E2 - B2) / B2c);
+    acc[4] += finite ? 0.0 : 1.0;
+  }
+  for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int m = blockDim.x / 2; m > 0; m >>= 1) {
+    if ((int)threadIdx.x < m) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + m];
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + m]);
+      sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + m]);
+      sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + m]);
+      sh[4][threadIdx.x] += sh[4][threadIdx.x + m];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NRED; q++) part[blockIdx.x * NRED + q] = sh[q][0];
+}
+
 }  // namespace coffee

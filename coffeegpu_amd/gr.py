@@ -18,6 +18,8 @@ from .solver import FieldSolverEZ, MultiArray, SimData, SimEnvironment, VectorFi
 
 
 class FieldSolverGREZ(FieldSolverEZ):
+    _energy_accounting = False
+
     def __init__(self, data: SimData, env: SimEnvironment, stream=None):
         self.m_data = data
         self.m_env = env

@@ -146,6 +146,18 @@ class SimData:
         self.P = MultiArray(g, dt, dev)
         self.divB = MultiArray(g, dt, dev)
         self.divE = MultiArray(g, dt, dev)
+        # energy accounting (sim_data.h:15-16): allocated by
+        # alloc_energy_accounting(), i.e. when a solver turns diagnostics on
+        self.dU_KO = self.dU_Epar = self.dU_EgtB = None
+        self.dU_KO_cum = self.dU_Epar_cum = self.dU_EgtB_cum = None
+
+    ACCOUNTING = ("dU_KO", "dU_Epar", "dU_EgtB", "dU_KO_cum", "dU_Epar_cum", "dU_EgtB_cum")
+
+    def alloc_energy_accounting(self):
+        g, dt, dev = self.env.grid(), self.env.dtype, self.env.device
+        for name in self.ACCOUNTING:
+            if getattr(self, name) is None:
+                setattr(self, name, MultiArray(g, dt, dev))
 
     def state_arrays(self):
         return [self.E.data(0), self.E.data(1), self.E.data(2), self.B.data(0), self.B.data(1),
@@ -202,6 +214,8 @@ class FieldSolverEZ:
     dP, Ptmp.  `mask_lanes` defaults to the reference CPU build's SIMD width
     for this Scalar (parity with the normative CPU semantics)."""
 
+    _energy_accounting = True  # the flat solver fills sim_data's dU_* arrays
+
     def __init__(self, data: SimData, env: SimEnvironment, mask_lanes: Optional[int] = None,
                  stream=None):
         torch = _torch()
@@ -254,10 +268,18 @@ class FieldSolverEZ:
         f.Ptmp = self.Ptmp.dev_ptr()
         f.divE = d.divE.dev_ptr()
         f.divB = d.divB.dev_ptr()
+        for name in SimData.ACCOUNTING:
+            arr = getattr(d, name, None)
+            setattr(f, name, arr.dev_ptr() if arr is not None else None)
         return f
 
     def set_diagnostics(self, on: bool):
+        """Diagnostics mode: J, divE, divB every stage and (flat solver) the
+        energy accounting arrays of sim_data, allocated on first use."""
         self._diag = bool(on)
+        if on and self._energy_accounting and hasattr(self.m_data, "alloc_energy_accounting"):
+            self.m_data.alloc_energy_accounting()
+            self._fields = self._make_fields()
         _lib.check(self._lib.coffee_set_diagnostics(self._ctx, int(on)), self._ctx)
 
     def evolve_fields(self, time: float):

@@ -77,6 +77,14 @@ class FieldSolverEZSpherical:
     def set_diagnostics(self, on: bool):
         _lib.check(self._lib.coffee_set_diagnostics(self._ctx, int(on)), self._ctx)
 
+    def residuals(self):
+        """Residual monitor of the state (metric dot products, sqrt(gamma)-
+        weighted div B): energy, max_divB, max_EdotB, max_E2mB2, nonfinite."""
+        out = (ctypes.c_double * 5)()
+        _lib.check(self._lib.coffee_residuals(self._ctx, ctypes.byref(self._fields), out,
+                                              self._s()), self._ctx)
+        return _residual_dict(out, 0)
+
     def set_timing(self, on: bool):
         _lib.check(self._lib.coffee_set_timing(self._ctx, int(on)), self._ctx)
 
@@ -90,6 +98,11 @@ class FieldSolverEZSpherical:
             self.close()
         except Exception:
             pass
+
+
+def _residual_dict(out, o):
+    return {"energy": out[o], "max_divB": out[o + 1], "max_EdotB": out[o + 2],
+            "max_E2mB2": out[o + 3], "nonfinite": out[o + 4]}
 
 
 def tile_shapes() -> List[tuple]:
@@ -177,6 +190,14 @@ class SphBatch:
             self._ctx, r.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
             self._s()), self._ctx)
         return out
+
+    def residuals(self) -> List[dict]:
+        """Residual monitor of every run (see FieldSolverEZSpherical.residuals)."""
+        n = self.n
+        out = (ctypes.c_double * (5 * n))()
+        _lib.check(self._lib.coffee_sph_batch_residuals(self._ctx, ctypes.cast(out, ctypes.c_void_p),
+                                                        self._s()), self._ctx)
+        return [_residual_dict(out, 5 * r) for r in range(n)]
 
     def status(self):
         act = (ctypes.c_int * self.n)()

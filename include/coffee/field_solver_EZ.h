@@ -123,6 +123,13 @@ class field_solver_EZ {
     return (Scalar)w;
   }
 
+  // Residual monitor (coffee_residuals): energy, max |div B|, max |E.B|/B^2,
+  // max (E^2 - B^2)/B^2, non-finite cells.
+  void residuals(double out[5]) {
+    coffee_fields f = fields();
+    coffee_check(coffee_residuals(m_ctx, &f, out, nullptr), "residuals", m_ctx);
+  }
+
   coffee_ctx* context() { return m_ctx; }
 
  private:
@@ -143,6 +150,12 @@ class field_solver_EZ {
     f.Ptmp = Ptmp.dev_ptr();
     f.divE = m_data.divE.dev_ptr();
     f.divB = m_data.divB.dev_ptr();
+    f.dU_KO = m_data.dU_KO.dev_ptr();
+    f.dU_Epar = m_data.dU_Epar.dev_ptr();
+    f.dU_EgtB = m_data.dU_EgtB.dev_ptr();
+    f.dU_KO_cum = m_data.dU_KO_cum.dev_ptr();
+    f.dU_Epar_cum = m_data.dU_Epar_cum.dev_ptr();
+    f.dU_EgtB_cum = m_data.dU_EgtB_cum.dev_ptr();
     return f;
   }
 

@@ -105,6 +105,14 @@ typedef struct coffee_fields {
   void *J[3];      /* sim_data::B0 (holds J) — written only when diagnostics are on */
   void *divE;      /* sim_data::divE — diagnostics */
   void *divB;      /* sim_data::divB — diagnostics */
+  /* sim_data energy accounting (field_solver_EZ.cpp:447-453,588-623), flat
+   * solver, diagnostics mode only, each optional: per step dU_KO (energy
+   * change of E, B by Kreiss-Oliger), dU_Epar, dU_EgtB (summed over the step's
+   * six projections; dU_EgtB counted from the pre-projection energy, so it
+   * includes dU_Epar, as the reference counts it), and the _cum arrays
+   * accumulating over all steps (never cleared by the solver). */
+  void *dU_KO, *dU_Epar, *dU_EgtB;
+  void *dU_KO_cum, *dU_Epar_cum, *dU_EgtB_cum;
 } coffee_fields;
 
 typedef struct coffee_ctx coffee_ctx;
@@ -146,10 +154,18 @@ int coffee_fill_guards(coffee_ctx *ctx, const coffee_fields *f, void *stream);
 int coffee_total_energy(coffee_ctx *ctx, void *const comp[3], double *out, void *stream);
 
 /* Residual monitor over interior cells (no reference counterpart; the
- * reference writes divE/divB/J arrays each stage instead): out[0] = field
- * energy sum(E^2+B^2), out[1] = max |div B|, out[2] = max |E.B|/max(B^2,TINY),
- * out[3] = max (E^2-B^2)/max(B^2,TINY) (> 0 means E > B somewhere),
- * out[4] = number of non-finite cells.  Synchronous on `stream`. */
+ * reference writes per-cell divE/divB/J arrays each stage instead,
+ * field_solver_EZ.cpp:123-125, field_solver_EZ_spherical.cu:325-329,
+ * field_solver_gr_EZ.cu:336-338): out[0] = field energy, out[1] = max |div B|,
+ * out[2] = max |E.B|/max(B^2,TINY), out[3] = max (E^2-B^2)/max(B^2,TINY)
+ * (> 0 means E > B somewhere), out[4] = number of cells with a non-finite
+ * E, B or P.  Flat: plain sums, sum(E^2+B^2).  Spherical and Kerr-Schild:
+ * dot products with the 3-metric (E_i B^i, ...), div B = d_i(sqrt(gamma) B^i)/
+ * sqrt(gamma) with the solver's 4th-order stencil, energy
+ * sum (E_i E^i + B_i B^i) sqrt(gamma) (times dx dtheta in the spherical case;
+ * axis cells, where sqrt(gamma) = 0, are left out of div B); for the GR
+ * solver E means D.  A spherical batch uses coffee_sph_batch_residuals.
+ * Synchronous on `stream`. */
 int coffee_residuals(coffee_ctx *ctx, const coffee_fields *f, double out[5], void *stream);
 
 /* Diagnostics mode: when on, each stage also writes J (into f->J), divE and
@@ -208,6 +224,9 @@ int coffee_sph_batch_autotune(coffee_batch *b, int trials, void *stream, int *be
                               double *ms_per_tile);
 /* Capture one step's launch sequence into a HIP graph and replay it. */
 int coffee_sph_batch_use_graph(coffee_batch *b, int enable);
+/* Residual monitor of every run (the quantities of coffee_residuals for the
+ * spherical solver), out[5*run .. 5*run+4].  Synchronous on `stream`. */
+int coffee_sph_batch_residuals(coffee_batch *b, double *out, void *stream);
 /* Per-run active flag, time and step count (synchronises the device). */
 int coffee_sph_batch_status(coffee_batch *b, int *active, double *t, int *step);
 /* Spin-down luminosity of every run through the sphere r = radius[run]:
@@ -226,6 +245,14 @@ int coffee_fill(void *p, unsigned long long n, int scalar_bytes, double value, v
  * / send_guard_cell_array callers (src/sim_env_impl.hpp:165-195). */
 int coffee_fill_guards_arrays(const coffee_grid *grid, const int periodic[3], void *const *arrays,
                               int narrays, int scalar_bytes, void *stream);
+
+/* Streaming copy of `bytes` (a multiple of 16, 16-byte aligned pointers)
+ * from src to dst, 16 bytes per lane, `unroll` (1, 2, 4, 8) loads in flight
+ * per lane, `blocks` workgroups of 256 lanes (asynchronous).  No reference
+ * counterpart: it measures the HBM bandwidth this device attains on a plain
+ * read + write stream, the ceiling the stencil kernels are compared with. */
+int coffee_stream_copy(const void *src, void *dst, unsigned long long bytes, int unroll,
+                       int blocks, void *stream);
 
 /* Last error message of this context (or of the last failed create when ctx
  * is NULL). */

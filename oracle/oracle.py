@@ -177,10 +177,19 @@ def reference_available(dtype=np.float64) -> bool:
     return os.path.exists(os.path.join(REF_DIR, name))
 
 
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def mpiexec_available() -> bool:
+    return os.path.exists(MPIEXEC)
+
+
 def run_reference(sp, U0: np.ndarray, nsteps: int, dumps: Sequence[int] = (),
-                  norms: bool = False, timing_warmup: int = -1, env=None):
+                  norms: bool = False, timing_warmup: int = -1, env=None, ranks: int = 1):
     """Run the reference CPU build (oracle/_ref) from U0 for nsteps.
 
+    ranks > 1 runs it under mpiexec with the z axis split over the ranks
+    (nodes = [1, 1, ranks]; timing only: no dumps or norms).
     Returns (dict step -> (7,k,j,i) array, norms array or None, stderr text)."""
     dtype = U0.dtype
     exe = os.path.join(REF_DIR, "ref_ez_f64" if dtype == np.float64 else "ref_ez_f32")
@@ -188,17 +197,24 @@ def run_reference(sp, U0: np.ndarray, nsteps: int, dumps: Sequence[int] = (),
         raise RuntimeError(f"reference harness not built: {exe}")
     tmp = tempfile.mkdtemp(prefix="coffee_ref_")
     try:
+        if ranks > 1:
+            sp = sp.replace(nodes=[1, 1, int(ranks)])
         with open(os.path.join(tmp, "config.toml"), "w") as f:
             f.write(sp.to_toml())
         write_cfld(os.path.join(tmp, "in.cfld"), U0)
         args = [exe, "in.cfld", "out", str(nsteps),
                 ",".join(str(s) for s in dumps) if dumps else "-",
                 "norms.txt" if norms else "-", str(timing_warmup)]
+        if ranks > 1:
+            args = [MPIEXEC, "-launcher", "fork", "-n", str(int(ranks))] + args
         r = subprocess.run(args, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                            text=True, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed ({r.returncode}): {r.stderr[-2000:]}")
         out = {s: read_cfld(os.path.join(tmp, f"out_s{s}.cfld")) for s in dumps}
+        # energy bookkeeping (dU_KO, dU_Epar, dU_EgtB, then the _cum arrays)
+        for s in dumps:
+            out[("dg", s)] = read_cfld(os.path.join(tmp, f"out_dg_s{s}.cfld"))
         nrm = np.loadtxt(os.path.join(tmp, "norms.txt"), ndmin=2) if norms else None
         return out, nrm, r.stderr
     finally:

@@ -47,6 +47,9 @@ multi_array<Scalar>* field_ptr(sim_data& d, int n) {
   return &d.P;
 }
 
+// Reads the whole-domain CFLD file; with several MPI ranks each rank keeps
+// the block of its own grid: local cell (i, j, k), guards included, is global
+// cell (i, j, k) + Grid::offset (src/sim_env_impl.hpp:155-160).
 bool read_cfld(const char* path, sim_data& d, const Grid& g) {
   FILE* f = fopen(path, "rb");
   if (!f) return false;
@@ -58,15 +61,24 @@ bool read_cfld(const char* path, sim_data& d, const Grid& g) {
     fprintf(stderr, "scalar size mismatch %d vs %d\n", hdr[1], (int)sizeof(Scalar));
     return false;
   }
+  const int gd[3] = {hdr[2], hdr[3], hdr[4]};
   for (int i = 0; i < 3; i++)
-    if (hdr[2 + i] != g.dims[i]) {
-      fprintf(stderr, "dims mismatch on axis %d: %d vs %d\n", i, hdr[2 + i], g.dims[i]);
+    if (g.offset[i] + g.dims[i] > gd[i]) {
+      fprintf(stderr, "block beyond the file on axis %d: %d + %d > %d\n", i, g.offset[i],
+              g.dims[i], gd[i]);
       return false;
     }
   if (hdr[5] != 7) return false;
-  size_t n = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+  const size_t ng = (size_t)gd[0] * gd[1] * gd[2];
+  std::vector<Scalar> buf(ng);
   for (int c = 0; c < 7; c++) {
-    if (fread(field_ptr(d, c)->host_ptr(), sizeof(Scalar), n, f) != n) return false;
+    if (fread(buf.data(), sizeof(Scalar), ng, f) != ng) return false;
+    Scalar* out = field_ptr(d, c)->host_ptr();
+    for (int k = 0; k < g.dims[2]; k++)
+      for (int j = 0; j < g.dims[1]; j++)
+        for (int i = 0; i < g.dims[0]; i++)
+          out[i + (j + (size_t)k * g.dims[1]) * g.dims[0]] =
+              buf[(i + g.offset[0]) + ((j + g.offset[1]) + (size_t)(k + g.offset[2]) * gd[1]) * gd[0]];
   }
   fclose(f);
   return true;
@@ -80,6 +92,22 @@ bool write_cfld(const std::string& path, sim_data& d, const Grid& g) {
   fwrite(hdr, sizeof(int), 6, f);
   size_t n = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
   for (int c = 0; c < 7; c++) fwrite(field_ptr(d, c)->host_ptr(), sizeof(Scalar), n, f);
+  fclose(f);
+  return true;
+}
+
+// The energy-accounting arrays of sim_data (field_solver_EZ.cpp:447-453,
+// 588-623) in the order dU_KO, dU_Epar, dU_EgtB and their _cum versions.
+bool write_diag_cfld(const std::string& path, sim_data& d, const Grid& g) {
+  multi_array<Scalar>* a[6] = {&d.dU_KO, &d.dU_Epar, &d.dU_EgtB,
+                               &d.dU_KO_cum, &d.dU_Epar_cum, &d.dU_EgtB_cum};
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return false;
+  int hdr[6] = {1, (int)sizeof(Scalar), g.dims[0], g.dims[1], g.dims[2], 6};
+  fwrite("CFLD", 1, 4, f);
+  fwrite(hdr, sizeof(int), 6, f);
+  size_t n = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+  for (int c = 0; c < 6; c++) fwrite(a[c]->host_ptr(), sizeof(Scalar), n, f);
   fclose(f);
   return true;
 }
@@ -151,6 +179,10 @@ int main(int argc, char* argv[]) {
     fprintf(stderr, "cannot read %s\n", in_path);
     return 3;
   }
+  if (env.size() > 1 && (!dumps.empty() || argc > 5 && strcmp(argv[5], "-") != 0)) {
+    fprintf(stderr, "dumps and norms are written by a single rank only\n");
+    return 5;
+  }
 
   field_solver_EZ solver(data, env);
   for (int i = 0; i < 3; i++) {
@@ -164,8 +196,10 @@ int main(int argc, char* argv[]) {
   Scalar time = 0.0;
   std::vector<double> step_ms;
   for (int step = 1; step <= nsteps; step++) {
+    if (env.size() > 1) MPI_Barrier(MPI_COMM_WORLD);
     auto t0 = std::chrono::steady_clock::now();
     solver.evolve_fields(time);
+    if (env.size() > 1) MPI_Barrier(MPI_COMM_WORLD);
     auto t1 = std::chrono::steady_clock::now();
     step_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
     time += env.params().dt;
@@ -178,7 +212,8 @@ int main(int argc, char* argv[]) {
     }
     if (dumps.count(step)) {
       std::string p = out_prefix + "_s" + std::to_string(step) + ".cfld";
-      if (!write_cfld(p, data, g)) {
+      std::string pd = out_prefix + "_dg_s" + std::to_string(step) + ".cfld";
+      if (!write_cfld(p, data, g) || !write_diag_cfld(pd, data, g)) {
         fprintf(stderr, "cannot write %s\n", p.c_str());
         return 4;
       }
@@ -190,8 +225,10 @@ int main(int argc, char* argv[]) {
     for (size_t i = warmup; i < step_ms.size(); i++) tot += step_ms[i];
     double ms = tot / (step_ms.size() - warmup);
     double cells = (double)(g.dims[0] - 2 * g.guard[0]) * (g.dims[1] - 2 * g.guard[1]) *
-                   (g.dims[2] - 2 * g.guard[2]);
-    fprintf(stderr, "TIMING ms_per_step=%.6f cell_updates_per_s=%.6e\n", ms, cells / ms * 1e3);
+                   (g.dims[2] - 2 * g.guard[2]) * env.size();
+    if (env.rank() == 0)
+      fprintf(stderr, "TIMING ms_per_step=%.6f cell_updates_per_s=%.6e ranks=%d\n", ms,
+              cells / ms * 1e3, env.size());
   }
   return 0;
 }

@@ -70,16 +70,20 @@ def case_pulsar(dtype, n=(32, 16, 16)):
     return sp, ic.stack(f)
 
 
-def case_alfven_boundary(dtype, n=(32, 16, 16), pert_type=2):
-    """problem 2: dipole star with a time-dependent twist (pert_type 2) or
-    angular-velocity pulse (pert_type 0)."""
+def case_alfven_boundary(dtype, n=(32, 16, 16), pert_type=2, **kw):
+    """problem 2: dipole star with a time-dependent twist (pert_type 2), an
+    axisymmetric angular-velocity pulse between the field lines through
+    rpert1 and rpert2 (pert_type 0, wpert, field_solver_EZ.cpp:644-661) or the
+    same pulse confined to ph1 pi < phi < ph2 pi (pert_type 1, wpert3d,
+    .cpp:663-684)."""
     sp = SimParams(dt=0.04, N=list(n), guard=[3, 3, 3], lower=[-2.0625, -1.0625, -1.0625],
                    size=[4.0, 2.0, 2.0], nodes=[1, 1, 1], periodic_boundary=[False] * 3,
                    pml=[0, 0, 0], pmllen=1, sigpml=10.0, divB_clean=True, ch2=1.0, tau=2.0,
                    KOeps=0.1, use_edotb_damping=False, problem=2, radius=0.6, omega=0.0,
                    b0=1.0, p1=0.0, p2=0.0, p3=1.0, tp_start=0.0, tp_end=1.0, dw0=0.4, nT=2.0,
                    theta0=0.3, drpert=0.5, rpert1=1.2, rpert2=3.0, pert_type=pert_type,
-                   KO_geometry=False)
+                   KO_geometry=False, ph1=0.2, ph2=0.8, dph=0.1)
+    sp = sp.replace(**kw) if kw else sp
     g = Grid.from_params(sp, dtype)
     f = ic.pulsar_dipole(g, sp, dtype=dtype)
     return sp, ic.stack(f)
@@ -91,6 +95,8 @@ CASES = {
     "noise": case_noise,
     "pulsar": case_pulsar,
     "alfven_bc": case_alfven_boundary,
+    "alfven_bc0": lambda dtype, **kw: case_alfven_boundary(dtype, pert_type=0, **kw),
+    "alfven_bc1": lambda dtype, **kw: case_alfven_boundary(dtype, pert_type=1, **kw),
 }
 
 # steps at which golden states are stored

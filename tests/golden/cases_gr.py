@@ -31,6 +31,7 @@ def case_gr(dtype, noise=1e-3, seed=7, **kw):
 
 GR_CASES = {
     "wald09": dict(a=0.9),
+    "wald099": dict(a=0.99),
     "wald05_ragged": dict(a=0.5, n=(28, 20, 24), lower=(-4.0, -3.0, -4.5),
                           size=(7.0, 5.0, 6.0), pml=(3, 2, 4)),
     "wald_a0_nopml": dict(a=0.0, pml=(0, 0, 0), noise=1e-2),

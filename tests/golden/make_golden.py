@@ -9,9 +9,12 @@ each Scalar type it stores, in tests/golden/flat_<case>_<dtype>.npz:
   - at the DUMPS steps: the 7 field values at 512 seeded sample cells
     (interior and guard cells alike).
 The 128^3 x 100-step wave (flat_c1_norms_f64.npz) stores the per-step norms
-and 512 sampled cells after step 100.
+and 512 sampled cells after step 100.  flat_diag_<case>_<dtype>.npz (cases
+DIAG_CASES) store the reference's per-cell energy bookkeeping (dU_KO,
+dU_Epar, dU_EgtB and their _cum arrays, field_solver_EZ.cpp:447-453,588-623)
+after steps 1 and DIAG_STEPS at 512 sampled interior cells.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [case ...]   (no case: every case and C1)
 """
 
 from __future__ import annotations
@@ -31,6 +34,9 @@ import oracle as O  # noqa: E402
 from tests.golden.cases import CASES, DUMPS, case_wave  # noqa: E402
 
 NSAMPLE = 512
+DIAG_CASES = ("noise", "pulsar")
+DIAG_DUMPS = (1, 3)
+DIAG_NAMES = ("dU_KO", "dU_Epar", "dU_EgtB", "dU_KO_cum", "dU_Epar_cum", "dU_EgtB_cum")
 
 
 def sample_index(shape, seed=12345):
@@ -39,12 +45,41 @@ def sample_index(shape, seed=12345):
     return np.sort(rng.choice(n, size=min(NSAMPLE, n), replace=False))
 
 
+def interior_sample_index(g, seed=54321):
+    k, j, i = np.meshgrid(*[np.arange(gd, d - gd) for gd, d in
+                            zip(g.guard[::-1], g.dims[::-1])], indexing="ij")
+    flat = (i + (j + k * g.dims[1]) * g.dims[0]).ravel()
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(flat, size=min(NSAMPLE, flat.size), replace=False))
+
+
+def diag_fixtures():
+    from coffeegpu_amd.params import Grid
+
+    for name in DIAG_CASES:
+        for dtype in (np.float64, np.float32):
+            sp, U0 = CASES[name](dtype)
+            out, _, _ = O.run_reference(sp, U0, max(DIAG_DUMPS), dumps=DIAG_DUMPS)
+            idx = interior_sample_index(Grid.from_params(sp, dtype))
+            rec = {"sha256": np.array(digest(U0)), "index": idx, "dumps": np.array(DIAG_DUMPS),
+                   "names": np.array(DIAG_NAMES)}
+            for s in DIAG_DUMPS:
+                rec[f"s{s}"] = out[("dg", s)].reshape(6, -1)[:, idx]
+            path = os.path.join(HERE, f"flat_diag_{name}_{np.dtype(dtype).name}.npz")
+            np.savez_compressed(path, **rec)
+            print("wrote", path)
+
+
 def digest(U: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(U).tobytes()).hexdigest()
 
 
-def main():
+def main(names=None):
+    if names and "diag" in names:
+        diag_fixtures()
     for name, fn in CASES.items():
+        if names and name not in names:
+            continue
         for dtype in (np.float64, np.float32):
             sp, U0 = fn(dtype)
             nsteps = max(DUMPS)
@@ -57,6 +92,8 @@ def main():
             path = os.path.join(HERE, f"flat_{name}_{np.dtype(dtype).name}.npz")
             np.savez_compressed(path, **rec)
             print("wrote", path)
+    if names:
+        return
     # C1 at full size: 128^3, FP64, 100 steps
     sp, U0 = case_wave(np.float64, n=(128, 128, 128), seed=1)
     out, nrm, _ = O.run_reference(sp, U0, 100, dumps=(100,), norms=True)
@@ -68,4 +105,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -118,3 +118,4 @@ def test_batch_entries_refuse_null_context():
     assert lib.coffee_sph_batch_set_tile(None, 0) == -4
     assert lib.coffee_sph_batch_use_graph(None, 1) == -4
     assert lib.coffee_sph_batch_autotune(None, 1, None, None, None) == -4
+

@@ -256,3 +256,37 @@ def test_schedules_bitwise_identical(name, dtype):
             idx = np.unravel_index(np.argmax(d), d.shape)
             print("max diff", d.max(), "at", idx, "value", outs[0][idx], "n differing", (d > 0).sum())
         assert np.array_equal(o, outs[0])
+
+
+ACC = ("dU_KO", "dU_Epar", "dU_EgtB", "dU_KO_cum", "dU_Epar_cum", "dU_EgtB_cum")
+
+
+@pytest.mark.parametrize("name", ["noise", "pulsar"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_energy_accounting_matches_oracle(name, dtype):
+    """dU_KO, dU_Epar, dU_EgtB and their _cum arrays after 3 steps in
+    diagnostics mode against the oracle (itself pinned to the reference
+    build's arrays by tests/golden/flat_diag_*.npz).  Per-step values are
+    differences of nearly equal energies: FP64 <= 1e-10 of the largest value,
+    FP32 <= 5e-4.  The state equals that of a run without diagnostics
+    bitwise."""
+    sp, U0 = CASES[name](dtype)
+    g = Grid.from_params(sp, dtype)
+    bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
+    solver, data = run_gpu(sp, U0, dtype, 3, bbg=bbg, diag=True)
+    lanes = solver.mask_lanes
+    got = {k: getattr(data, k).sync_to_host().copy() for k in ACC}
+    state = data.get_state().copy()
+    solver.close()
+    plain, pdata = run_gpu(sp, U0, dtype, 3, bbg=bbg)
+    assert np.array_equal(state, pdata.get_state())
+    plain.close()
+    ref = {k: np.zeros(U0.shape[1:], dtype) for k in ACC}
+    run_oracle(sp, U0, dtype, 3, bbg=bbg, mask_lanes=lanes, diag=ref)
+    it = g.interior()
+    tol = 1e-10 if dtype == np.float64 else 5e-4
+    for k in ACC:
+        a, b = got[k][it].astype(np.float64), ref[k][it].astype(np.float64)
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        print(f"{name} {np.dtype(dtype).name} {k}: {err:.2e} (max {np.abs(b).max():.2e})")
+        assert err <= tol, k

@@ -121,11 +121,12 @@ def _gr_run(sp, fields, variant):
     return out
 
 
-def test_gr_384_schedules_bitwise():
-    sp = bench.gr_wald_params(GR_N, 0.9)
+@pytest.mark.parametrize("a", [0.5, 0.9, 0.99])
+def test_gr_384_schedules_bitwise(a):
+    sp = bench.gr_wald_params(GR_N, a)
     grid = SimEnvironment(sp, np.float32, "cuda:0").grid()
     fields = ic.wald_uniform(grid, sp.a, noise=1e-3, seed=0, dtype=np.float32)
-    print(f"gr {GR_N}^3 fp32 wald a=0.9, {GR_STEPS} steps: tiled (default)", flush=True)
+    print(f"gr {GR_N}^3 fp32 wald a={a}, {GR_STEPS} steps: tiled (default)", flush=True)
     tiled = _gr_run(sp, fields, 0)
     assert all(bool(torch.isfinite(a).all()) for a in tiled)
     print("gr: point sweep + stage sweep schedule", flush=True)

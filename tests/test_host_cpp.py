@@ -112,3 +112,73 @@ def test_cpp_driver_matches_oracle(dtype):
     err = O.rel_l2(got, st.U)
     print(f"C++ driver vs oracle ({np.dtype(dtype).name}, pulsar, 5 steps): {err:.2e}")
     assert err <= (1e-12 if dtype == np.float64 else 1e-5)
+
+
+def _run_driver(dtype, sp, U0, nsteps, kind):
+    import oracle as O
+
+    exe = os.path.join(ROOT, "coffeegpu_amd", "bin",
+                       "coffee_drive_f64" if dtype == np.float64 else "coffee_drive_f32")
+    assert os.path.exists(exe), "C++ driver not built"
+    tmp = tempfile.mkdtemp(prefix="coffee_drive_")
+    try:
+        with open(os.path.join(tmp, "config.toml"), "w") as f:
+            f.write(sp.to_toml())
+        O.write_cfld(os.path.join(tmp, "in.cfld"), U0)
+        r = subprocess.run([exe, "in.cfld", "out.cfld", str(nsteps), kind], cwd=tmp,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        return O.read_cfld(os.path.join(tmp, "out.cfld")), r.stdout
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cpp_spherical_solver_matches_oracle(dtype):
+    """field_solver_EZ_spherical through the C++ surface (config.toml ->
+    sim_environment -> sim_data -> solver.evolve_fields) against the oracle."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import oracle as O
+    from tests.golden.cases_sph import case_sph
+
+    sp, U0 = case_sph(dtype, omega=0.3)
+    got, out = _run_driver(dtype, sp, U0, 5, "sph")
+    st = O.SphState(U0, dtype)
+    O.sph_evolve(sp, st, 5)
+    err = O.rel_l2(got, st.U)
+    print(f"C++ spherical solver vs oracle ({np.dtype(dtype).name}, 5 steps): {err:.2e}\n{out}")
+    assert err <= (1e-12 if dtype == np.float64 else 1e-5)
+    assert "nonfinite 0" in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cpp_gr_solver_matches_oracle(dtype):
+    """field_solver_gr_EZ through the C++ surface against the FP64 oracle
+    (FP32: within 1e-5 outside the horizon, the bar of the GR FP32 tests)."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import oracle as O
+    from coffeegpu_amd import initial_data as ic
+    from coffeegpu_amd.params import Grid
+    from tests.golden.cases_gr import case_gr
+
+    sp, U0 = case_gr(np.float64, a=0.9, noise=0.0 if dtype == np.float32 else 1e-3)
+    got, out = _run_driver(dtype, sp, U0.astype(dtype), 5, "gr")
+    st = O.GrState(U0)
+    O.gr_evolve(sp, st, 5)
+    if dtype == np.float64:
+        err = O.rel_l2(got, st.U)
+    else:
+        g = Grid.from_params(sp, np.float64)
+        X, Y, Z = (np.array(v) for v in np.broadcast_arrays(*g.coords()))
+        outside = ic.cks_metric_np(sp.a, X, Y, Z)[0] >= 1 + np.sqrt(1 - sp.a ** 2)
+        d = got.astype(np.float64) - st.U
+        err = float(np.sqrt((d[:, outside] ** 2).sum() / (st.U[:, outside] ** 2).sum()))
+    print(f"C++ GR solver vs oracle ({np.dtype(dtype).name}, 5 steps): {err:.2e}\n{out}")
+    assert err <= (1e-12 if dtype == np.float64 else 1e-5)
+    assert "nonfinite 0" in out

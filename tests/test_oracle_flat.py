@@ -266,3 +266,70 @@ def test_oracle_matches_reference_build_live():
     st = O.FlatState(U0, np.float64)
     O.flat_evolve(sp, st, 3, mask_lanes=8)
     assert O.rel_l2(st.U, ref[3]) <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["alfven_bc0", "alfven_bc1"])
+@pytest.mark.parametrize("mistake", [{"dw0": 0.0}, {"rpert2": 2.0}, {"nT": 1.0}])
+def test_alfven_pulse_is_pinned_by_its_fixture(name, mistake):
+    """The angular-velocity pulse of boundary_alfven (pert_type 0: wpert,
+    pert_type 1: wpert3d, field_solver_EZ.cpp:644-684,887-900) shapes the
+    fixtures: no pulse, another field-line band or another pulse count fails
+    the comparison with the reference build."""
+    worst, nrel = compare_to_gold(name, np.float64, sp_override=mistake)
+    assert worst > 1e-10 or nrel > 1e-10
+
+
+DIAG_NAMES = ("dU_KO", "dU_Epar", "dU_EgtB", "dU_KO_cum", "dU_Epar_cum", "dU_EgtB_cum")
+
+
+def oracle_diag(name, dtype, steps):
+    sp, U0 = CASES[name](dtype)
+    g = Grid.from_params(sp, dtype)
+    d = {k: np.zeros(g.shape, dtype) for k in DIAG_NAMES}
+    bbg = O.flat_bbg(sp, dtype) if any(sp.pml) else None
+    st = O.FlatState(U0, dtype, bbg)
+    out = {}
+    for s in range(1, steps + 1):
+        O.flat_evolve(sp, st, 1, time=(s - 1) * sp.dt, mask_lanes=reference_simd_lanes(dtype),
+                      diag=d)
+        out[s] = np.stack([d[k].copy() for k in DIAG_NAMES])
+    return out, U0
+
+
+def diag_worst(name, dtype, mutate=None):
+    gold = np.load(os.path.join(GOLD, f"flat_diag_{name}_{np.dtype(dtype).name}.npz"))
+    got, U0 = oracle_diag(name, dtype, int(max(gold["dumps"])))
+    assert hashlib.sha256(np.ascontiguousarray(U0).tobytes()).hexdigest() == str(gold["sha256"])
+    worst = 0.0
+    for s in gold["dumps"]:
+        a = got[int(s)].reshape(6, -1)[:, gold["index"]].astype(np.float64)
+        if mutate:
+            a = mutate(a)
+        b = gold[f"s{s}"].astype(np.float64)
+        for q in range(6):
+            worst = max(worst, np.abs(a[q] - b[q]).max() / max(np.abs(b[q]).max(), 1e-300))
+    return worst
+
+
+@pytest.mark.parametrize("name", ["noise", "pulsar"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_energy_bookkeeping_matches_reference_build(name, dtype):
+    """dU_KO, dU_Epar, dU_EgtB and their _cum arrays (field_solver_EZ.cpp:
+    447-453, 588-623: the E>B part counted from the pre-projection energy, so
+    it includes the E.B part) against the reference build after 1 and 3
+    steps.  FP32 per-step values are differences of nearly equal energies:
+    their relative error is that of the cancellation."""
+    worst = diag_worst(name, dtype)
+    print(f"{name} {np.dtype(dtype).name}: {worst:.2e}")
+    assert worst <= (1e-12 if dtype == np.float64 else 5e-4)
+
+
+def test_energy_bookkeeping_fixture_catches_single_counting():
+    """Counting the E>B change from the post-E.B energy (no double count)
+    fails the fixture."""
+    def single(a):
+        a = a.copy()
+        a[2] = a[2] - a[1]
+        a[5] = a[5] - a[4]
+        return a
+    assert diag_worst("noise", np.float64, mutate=single) > 1e-6
