@@ -131,6 +131,9 @@ SIGNATURES = {
                                    ctypes.c_double, ctypes.c_void_p]),
     "coffee_stream_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "coffee_downsample": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
     "coffee_fill_guards_arrays": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid), ctypes.c_int * 3,
                                                  ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_void_p]),

@@ -1,8 +1,9 @@
 """Build recipe for libcoffee_hip.so (gfx950) — explicit hipcc, in-tree.
 
 Each HIP translation unit compiles to its own object (in parallel, under
-_build/), then one hipcc link makes the shared library; the C++ drivers link
-against it.
+_build/), then one hipcc link makes the shared library; libcoffee_h5.so (the
+HDF5 file layer, host code) is built beside it; the C++ drivers link against
+both.
 """
 
 from __future__ import annotations
@@ -18,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcoffee_hip.so")
 OBJDIR = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip", "unet.hip"]
+HIP_SOURCES = ["coffee_hip.hip", "coffee_sph.hip", "coffee_gr.hip", "unet.hip", "coffee_io.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 # -ffp-contract=off: every fused multiply-add is written out as fma() in the
@@ -37,7 +38,9 @@ def hipcc() -> str:
 
 # (name, source, extra flags): C++ drivers over the host surface
 DRIVERS = [("coffee_drive_f32", "drivers/coffee_drive.cpp", []),
-           ("coffee_drive_f64", "drivers/coffee_drive.cpp", ["-DUSE_DOUBLE"])]
+           ("coffee_drive_f64", "drivers/coffee_drive.cpp", ["-DUSE_DOUBLE"]),
+           ("coffee_restart_f32", "drivers/coffee_restart.cpp", ["-lcoffee_h5"]),
+           ("coffee_restart_f64", "drivers/coffee_restart.cpp", ["-DUSE_DOUBLE", "-lcoffee_h5"])]
 
 
 def _run(cmd, what: str, verbose: bool):
@@ -63,15 +66,44 @@ def build_drivers(verbose: bool = False):
     def one(d):
         name, src, extra = d
         out = os.path.join(bindir, name)
-        cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include")] + extra + [
-            "-o", out + ".tmp", os.path.join(CSRC, src), "-L" + HERE, "-lcoffee_hip",
+        flags = [e for e in extra if not e.startswith("-l")]
+        libs = [e for e in extra if e.startswith("-l")]
+        cmd = [hipcc(), "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include")] + flags + [
+            "-o", out + ".tmp", os.path.join(CSRC, src), "-L" + HERE, "-lcoffee_hip"] + libs + [
             "-Wl,-rpath,$ORIGIN/.."]
         _run(cmd, name, verbose)
         os.replace(out + ".tmp", out)
 
-    todo = [d for d in DRIVERS if os.path.exists(os.path.join(CSRC, d[1]))]
+    todo = [d for d in DRIVERS if os.path.exists(os.path.join(CSRC, d[1]))
+            and ("-lcoffee_h5" not in d[2] or os.path.exists(H5_OUT))]
     with ThreadPoolExecutor(_jobs()) as ex:
         list(ex.map(one, todo))
+
+
+# HDF5 C library for the snapshot / field-output files (libcoffee_h5.so)
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
+H5_OUT = os.path.join(HERE, "libcoffee_h5.so")
+
+
+def build_h5(verbose: bool = False):
+    """libcoffee_h5.so: the HDF5 file layer (host code, g++), linked to the
+    serial HDF5 C library under HDF5_ROOT with a run-time path to it (the C++
+    runtime linked in statically and not exported, so that path cannot hand
+    an older libstdc++ from the same directory to a C++ host).  When
+    the HDF5 headers are absent it is not built, and the exporter raises when
+    used."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"[build] no hdf5.h under {inc}: libcoffee_h5.so not built", file=sys.stderr)
+        return None
+    tmp = H5_OUT + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          "-I" + inc, "-o", tmp, os.path.join(CSRC, "io", "coffee_h5.cpp"), "-L" + lib, "-lhdf5",
+          "-static-libstdc++", "-static-libgcc", "-Wl,--exclude-libs,ALL",
+          "-Wl,--enable-new-dtags,-rpath," + lib], "libcoffee_h5.so", verbose)
+    os.replace(tmp, H5_OUT)
+    return H5_OUT
 
 
 def build(verbose: bool = False) -> str:
@@ -92,6 +124,7 @@ def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, "link", verbose)
     os.replace(tmp, OUT)
+    build_h5(verbose)
     build_drivers(verbose)
     return OUT
 

@@ -50,6 +50,20 @@ class SimEnvironment:
     def grid(self) -> Grid:
         return self._grid
 
+    def send_guard_cells(self, data: "SimData", stream=None):
+        """sim_environment::send_guard_cells(sim_data&) on one rank
+        (src/sim_env_impl.hpp:179-195, src/sim_env.cpp:365-438): periodic
+        axes take the interior one period away (x, then y, then z) in E, B
+        and P; other axes keep their guard cells."""
+        torch = _torch()
+        g = coffee_grid(self._grid)
+        per = (ctypes.c_int * 3)(*[int(bool(v)) for v in self._params.periodic_boundary])
+        arrs = (ctypes.c_void_p * 7)(*[a.dev_ptr() for a in data.state_arrays()])
+        s = stream if stream is not None else torch.cuda.current_stream()
+        lib = _lib.load()
+        _lib.check(lib.coffee_fill_guards_arrays(ctypes.byref(g), per, arrs, 7,
+                                                 self.dtype.itemsize, ctypes.c_void_p(s.cuda_stream)))
+
     def rank(self) -> int:
         return 0
 

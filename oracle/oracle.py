@@ -426,3 +426,80 @@ def gr_absorb(sp, U: np.ndarray, Un: np.ndarray) -> np.ndarray:
     lib().ffe_gr_absorb_f64(ctypes.byref(make_params(sp)), _ptrs(list(U), ctypes.c_double),
                             _ptrs(list(Un), ctypes.c_double))
     return U
+
+
+# ---- field-output sampling -------------------------------------------------
+
+def downsample_extent(dims, guard, d: int):
+    """Extent of the sampled output: N / d where N > d, else N
+    (data_exporter.cpp:70-79)."""
+    out = []
+    for n, g in zip(dims, guard):
+        m = n - 2 * g
+        out.append(m // d if m > d else m)
+    return tuple(out)
+
+
+def downsample(arr: np.ndarray, guard, stagger: int, d: int) -> np.ndarray:
+    """Point sample every d-th interior cell, interpolated from the array's
+    stagger to the cell centre (Stagger 0b111), in the array's Scalar type,
+    returned as float32 of the sampled extent (Fortran order flattened as
+    arr is).  3D arrays: multi_array::downsample (src/data/multi_array.cpp:
+    56-73) with interpolate (src/algorithms/interpolation.h:9-34); 2D arrays
+    (dims[2] == 1): the device path's downsample2d with interpolate2d
+    (src/cuda/data/multi_array.cu:73-92, interpolation.h:36-50).
+
+    `arr` has shape (dims[2], dims[1], dims[0]) (C order of a Fortran-order
+    array).  With out = 0b111 the minus offset of an axis is -1 where the
+    stagger bit is 0 and 0 where it is 1; the plus offset is always 0.
+
+    An axis with 1 < N <= d keeps its N cells, sampled with stride 1: the
+    reference keeps the extent N there but still steps by d, reading beyond
+    the array (multi_array.cpp:62-66)."""
+    T = arr.dtype.type
+    dz, dy, dx = arr.shape
+    half = T(0.5)
+    ex = downsample_extent((dx, dy, dz), guard, d)
+    sd = [d if n - 2 * g > d else 1 for n, g in zip((dx, dy, dz), guard)]
+    i = np.arange(ex[0]) * sd[0] + guard[0]
+    j = np.arange(ex[1]) * sd[1] + guard[1]
+    mi = -1 if not (stagger & 1) else 0
+    mj = -1 if not (stagger & 2) else 0
+    if dz == 1:
+        kk = np.zeros(1, dtype=int)
+        J, I = np.meshgrid(j, i, indexing="ij")
+        p = arr[0]
+        f1 = half * (p[J, I] + p[J + mj, I])
+        f0 = half * (p[J, I + mi] + p[J + mj, I + mi])
+        return (half * (f1 + f0)).astype(np.float32)[None]
+    k = np.arange(ex[2]) * sd[2] + guard[2]
+    mk = -1 if not (stagger & 4) else 0
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    f11 = half * (arr[K + mk, J, I] + arr[K, J, I])
+    f10 = half * (arr[K + mk, J + mj, I] + arr[K, J + mj, I])
+    f01 = half * (arr[K + mk, J, I + mi] + arr[K, J, I + mi])
+    f00 = half * (arr[K + mk, J + mj, I + mi] + arr[K, J + mj, I + mi])
+    f1 = half * (f11 + f10)
+    f0 = half * (f01 + f00)
+    return (half * (f1 + f0)).astype(np.float32)
+
+
+def reference_downsample(arr: np.ndarray, guard, stagger: int, d: int) -> np.ndarray:
+    """The reference build's own multi_array::downsample (3D arrays), run in
+    oracle/_ref/ref_downsample_f32|f64 on `arr` (shape (dz, dy, dx))."""
+    exe = os.path.join(REF_DIR, "ref_downsample_f64" if arr.dtype == np.float64
+                       else "ref_downsample_f32")
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    dz, dy, dx = arr.shape
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([arr.dtype.itemsize, dx, dy, dz, *guard, stagger, d],
+                     dtype="<i4").tofile(f)
+            np.ascontiguousarray(arr).tofile(f)
+        subprocess.run([exe, fin, fout], check=True)
+        raw = np.fromfile(fout, dtype="<i4", count=3)
+        out = np.fromfile(fout, dtype="<f4", offset=12)
+    ex = tuple(int(v) for v in raw)
+    return out.reshape(ex[2], ex[1], ex[0])

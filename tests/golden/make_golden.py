@@ -14,6 +14,9 @@ DIAG_CASES) store the reference's per-cell energy bookkeeping (dU_KO,
 dU_Epar, dU_EgtB and their _cum arrays, field_solver_EZ.cpp:447-453,588-623)
 after steps 1 and DIAG_STEPS at 512 sampled interior cells.
 
+downsample_<dtype>.npz hold the reference build's field-output sampling
+(multi_array::downsample) of seeded arrays ("downsample" case name).
+
 Usage: python tests/golden/make_golden.py [case ...]   (no case: every case and C1)
 """
 
@@ -70,6 +73,32 @@ def diag_fixtures():
             print("wrote", path)
 
 
+# every interior extent above the largest factor: where 1 < N <= d the
+# reference steps beyond the array (multi_array.cpp:62-66)
+DS_SHAPES = [((14, 12, 10), (3, 3, 3)), ((17, 11, 13), (3, 2, 3))]
+DS_STAGGERS = [0b110, 0b101, 0b011, 0b001, 0b010, 0b100, 0b111]
+DS_FACTORS = [1, 2, 3]
+
+
+def downsample_fixtures():
+    """downsample_<dtype>.npz: the reference build's multi_array::downsample
+    (oracle/_ref/ref_downsample_*) of seeded arrays, for every E/B stagger
+    and the cell centre, d = 1, 2, 3."""
+    for dtype in (np.float64, np.float32):
+        rng = np.random.default_rng(2024)
+        rec = {}
+        for s, (dims, guard) in enumerate(DS_SHAPES):
+            a = rng.standard_normal((dims[2], dims[1], dims[0])).astype(dtype)
+            rec[f"in{s}"] = a
+            rec[f"guard{s}"] = np.array(guard)
+            for st in DS_STAGGERS:
+                for d in DS_FACTORS:
+                    rec[f"out{s}_{st}_{d}"] = O.reference_downsample(a, guard, st, d)
+        path = os.path.join(HERE, f"downsample_{np.dtype(dtype).name}.npz")
+        np.savez_compressed(path, **rec)
+        print("wrote", path)
+
+
 def digest(U: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(U).tobytes()).hexdigest()
 
@@ -77,6 +106,8 @@ def digest(U: np.ndarray) -> str:
 def main(names=None):
     if names and "diag" in names:
         diag_fixtures()
+    if names and "downsample" in names:
+        downsample_fixtures()
     for name, fn in CASES.items():
         if names and name not in names:
             continue

@@ -9,11 +9,19 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+# headers of libcoffee_h5.so; every other include/*.h is libcoffee_hip.so's
+H5_HEADERS = ("coffee_h5.h",)
+
+
+def declared_symbols(headers=None):
+    """coffee_* functions declared by include/*.h (all but the HDF5 layer's
+    headers), or by the named headers."""
     names = set()
     inc = os.path.join(ROOT, "include")
     for fn in os.listdir(inc):
         if not fn.endswith(".h"):
+            continue
+        if (headers is None and fn in H5_HEADERS) or (headers is not None and fn not in headers):
             continue
         text = open(os.path.join(inc, fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -38,6 +46,19 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # ctypes signatures cover every declared symbol
     assert declared_symbols() <= set(_lib.SIGNATURES)
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from coffeegpu_amd import _h5
+
+    if not os.path.exists(_h5.LIB_PATH):
+        pytest.skip("libcoffee_h5.so not built (no HDF5 at build time)")
+    lib = ctypes.CDLL(_h5.LIB_PATH)
+    names = declared_symbols(H5_HEADERS)
+    assert "coffee_h5_create" in names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names == set(_h5.SIGNATURES)
 
 
 def test_abi_version_and_create_error_without_gpu():
