@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "libcoffee_hip.so")
 COFFEE_SOLVER_EZ = 0
 COFFEE_SOLVER_EZ_SPHERICAL = 1
 COFFEE_SOLVER_GR_EZ = 2
+# static partner-state array of coffee_ez_step_noalloc (COFFEE_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 80 << 20
 
 _D3 = ctypes.c_double * 3
 _I3 = ctypes.c_int * 3
@@ -75,6 +77,10 @@ SIGNATURES = {
     "coffee_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CoffeeParams)]),
     "coffee_ez_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CoffeeFields),
                                       ctypes.c_double, ctypes.c_void_p]),
+    "coffee_ez_step_noalloc": (ctypes.c_int, [ctypes.POINTER(CoffeeGrid),
+                                              ctypes.POINTER(CoffeeParams), ctypes.c_int,
+                                              ctypes.POINTER(CoffeeFields), ctypes.c_double,
+                                              ctypes.c_void_p]),
     "coffee_fill_guards": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CoffeeFields),
                                           ctypes.c_void_p]),
     "coffee_total_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p * 3,

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -420,6 +421,32 @@ int energy_impl(coffee_ctx* c, void* const comp[3], double* out, hipStream_t s) 
   return COFFEE_OK;
 }
 
+// Partner state of coffee_ez_step_noalloc.  Host code never uses this name
+// as an address: each device's copy is reached through the pointer that
+// hipGetSymbolAddress returns with that device current (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[COFFEE_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev`, looked up on the
+// first call on that device and kept for the process.
+int noalloc_scratch(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, COFFEE_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, COFFEE_ERR_HIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return COFFEE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -522,6 +549,41 @@ int coffee_ez_step(coffee_ctx* c, const coffee_fields* f, double time, void* str
   if (c->solver == COFFEE_SOLVER_EZ_SPHERICAL) return coffee::sph_step(c, f, time, s);
   if (c->solver == COFFEE_SOLVER_GR_EZ) return coffee::gr_step(c, f, time, s);
   return c->scalar == 4 ? ez_step_impl<float>(c, f, time, s) : ez_step_impl<double>(c, f, time, s);
+}
+
+int coffee_ez_step_noalloc(const coffee_grid* grid, const coffee_params* params, int scalar_bytes,
+                           const coffee_fields* f, double time, void* stream) {
+  if (!grid || !params || !f) return fail(nullptr, COFFEE_ERR_INVALID, "null argument");
+  if (scalar_bytes != COFFEE_F32 && scalar_bytes != COFFEE_F64)
+    return fail(nullptr, COFFEE_ERR_INVALID, "scalar_bytes must be 4 or 8");
+  int rc = validate(nullptr, grid, params);
+  if (rc) return rc;
+  const unsigned long long per = (cells(*grid) * (unsigned long long)scalar_bytes + 255ull) & ~255ull;
+  if (7ull * per > COFFEE_NOALLOC_SCRATCH_BYTES)
+    return fail(nullptr, COFFEE_ERR_SIZE,
+                "coffee_ez_step_noalloc: the partner state (7 x " + std::to_string(per) +
+                    " bytes) exceeds COFFEE_NOALLOC_SCRATCH_BYTES (" +
+                    std::to_string(COFFEE_NOALLOC_SCRATCH_BYTES) + ")");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, COFFEE_ERR_HIP, "hipGetDevice failed");
+  void* base = nullptr;
+  rc = noalloc_scratch(dev, &base);
+  if (rc) return rc;
+  // A context on the stack: the parameter blocks and the partner pointers,
+  // no device allocation, timing and diagnostics off, default schedule.
+  coffee_ctx c;
+  c.solver = COFFEE_SOLVER_EZ;
+  c.scalar = scalar_bytes;
+  c.device = dev;
+  c.grid = *grid;
+  c.params = *params;
+  fill_k<float>(*grid, *params, c.kf);
+  fill_k<double>(*grid, *params, c.kd);
+  for (int q = 0; q < 7; q++) c.scratch[q] = (unsigned char*)base + q * per;
+  hipStream_t s = (hipStream_t)stream;
+  rc = scalar_bytes == 4 ? ez_step_impl<float>(&c, f, time, s) : ez_step_impl<double>(&c, f, time, s);
+  if (rc) return fail(nullptr, rc, c.err);
+  return COFFEE_OK;
 }
 
 int coffee_fill_guards(coffee_ctx* c, const coffee_fields* f, void* stream) {

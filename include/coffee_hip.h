@@ -143,6 +143,29 @@ int coffee_set_params(coffee_ctx *ctx, const coffee_params *params);
  * when an absorbing layer is active. */
 int coffee_ez_step(coffee_ctx *ctx, const coffee_fields *f, double time, void *stream);
 
+/* Largest partner state coffee_ez_step_noalloc accepts: 7 arrays of
+ * dims[0]*dims[1]*dims[2] scalars, each rounded up to 256 bytes (80 MiB: a
+ * 128^3 FP32 grid with guard 3 needs 67.4 MB; FP64 fits up to about
+ * 1.49e6 cells including guards, e.g. 108^3 interior with guard 3). */
+#define COFFEE_NOALLOC_SCRATCH_BYTES (80ull << 20)
+
+/* No counterpart in the reference.  coffee_ez_step without a context and
+ * without any allocation, for callers that cannot call hipMalloc (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation):
+ * the same kernels and launch sequence as coffee_ez_step on a fresh flat
+ * context (default schedule, diagnostics and timing off), so the results are
+ * bitwise those of coffee_ez_step.  The ping-pong partner state lives in one
+ * static device array of COFFEE_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; grids needing more return COFFEE_ERR_SIZE.  Nothing in it
+ * has to start at zero (every step writes the partner's interior and copies
+ * its frozen guards first).  All calls on one GPU share that array: calls
+ * that overlap on one GPU (two streams or two threads) race and must be
+ * serialised by the caller.  Arguments as coffee_ctx_create + coffee_ez_step
+ * with solver COFFEE_SOLVER_EZ. */
+int coffee_ez_step_noalloc(const coffee_grid *grid, const coffee_params *params,
+                           int scalar_bytes, const coffee_fields *f, double time,
+                           void *stream);
+
 /* Guard-cell fill on one GPU: periodic axes take the interior one period
  * away (x, then y, then z); other axes are left untouched (frozen), as
  * send_guard_cells does on a single rank (src/sim_env.cpp:365-438). */
