@@ -80,8 +80,8 @@ GrK<float>& grk<float>(GrState* S) { return S->kf; }
 template <>
 GrK<double>& grk<double>(GrState* S) { return S->kd; }
 
-// Guard slabs on every axis whose periodicity matches `wrap` (src -> dst),
-// using the flat solver's slab kernel: same layout and exchange order.
+// Guard cells with the flat solver's kernels: periodic ones in place in one
+// pass (wrap, dst == src), frozen ones copied slab by slab (src -> dst).
 template <typename T>
 int gr_guards(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bool wrap) {
   const FlatK<T>& k = flat_k<T>(c);
@@ -89,6 +89,12 @@ int gr_guards(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], bo
   for (int q = 0; q < 7; q++) {
     d.p[q] = dst[q];
     r.p[q] = src[q];
+  }
+  if (wrap) {
+    if (!k.per[0] && !k.per[1] && !k.per[2]) return COFFEE_OK;
+    const long total = (long)k.dims[0] * k.dims[1] * k.dims[2] - (long)k.n[0] * k.n[1] * k.n[2];
+    const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+    return timed(c, "guard_fill", s, [&] { guard_wrap_kernel<T><<<blocks, 256, 0, s>>>(k, d, 7); });
   }
   for (int ax = 0; ax < 3; ax++) {
     if (wrap != (bool)k.per[ax]) continue;

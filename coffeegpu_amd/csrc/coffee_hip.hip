@@ -161,6 +161,15 @@ int guard_slabs(coffee_ctx* c, hipStream_t s, T* const dst[7], T* const src[7], 
     d.p[q] = dst[q];
     r.p[q] = src[q];
   }
+  if (wrap) {
+    // in place (dst == src): every periodic guard cell in one launch
+    if (!k.per[0] && !k.per[1] && !k.per[2]) return COFFEE_OK;
+    const long total = (long)k.dims[0] * k.dims[1] * k.dims[2] - (long)k.n[0] * k.n[1] * k.n[2];
+    const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+    return timed(c, "guard_fill", s, [&] {
+      guard_wrap_kernel<T><<<blocks, 256, 0, s>>>(k, d, nfields);
+    });
+  }
   for (int ax = 0; ax < 3; ax++) {
     if (k.guard[ax] == 0) continue;
     if (wrap != (bool)k.per[ax]) continue;
@@ -242,9 +251,22 @@ void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim
   }
 }
 
+// Buffers of P over the five stages: A -> B -> C -> B -> C -> A (fields 0-5
+// alternate A -> B -> A -> B -> A -> B and Kreiss-Oliger brings them back to
+// A).  Every stage reads P from another buffer than it writes, and the last
+// stage writes P into the caller's array, so Kreiss-Oliger, which does not
+// change P, neither reads nor writes it.
+template <typename T>
+void p_buffers(int st, T* a, T* b, T* cb, FlatIO<T>& io) {
+  static const int from[5] = {0, 1, 2, 1, 2}, to[5] = {1, 2, 1, 2, 0};
+  T* const buf[3] = {a, b, cb};
+  io.in[6] = buf[from[st]];
+  io.out[6] = buf[to[st]];
+}
+
 template <typename T, int TX, int TY>
 int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7], T* const B[7],
-                  double time, hipStream_t s, bool star) {
+                  T* const Cp, double time, hipStream_t s, bool star) {
   static const double As[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471,
                                -1.514183444257};
   static const double Bs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869,
@@ -268,10 +290,11 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
   for (int st = 0; st < 5; st++) {
     T* const* in = (st % 2 == 0) ? A : B;
     T* const* out = (st % 2 == 0) ? B : A;
-    for (int q = 0; q < 7; q++) {
+    for (int q = 0; q < 6; q++) {
       io.in[q] = in[q];
       io.out[q] = out[q];
     }
+    p_buffers<T>(st, A[6], B[6], Cp, io);
     const T as = (T)As[st], bs = (T)Bs[st];
     const T ts = t0 + (T)cs[st] * dt;
     const char* name = st == 0 ? "ez_stage_first" : (st == 4 ? "ez_stage_last" : "ez_stage");
@@ -285,11 +308,12 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
     });
     if (rc) return rc;
   }
-  // Kreiss-Oliger: B -> A
-  for (int q = 0; q < 7; q++) {
+  // Kreiss-Oliger on E, B: B -> A (P is already in A)
+  for (int q = 0; q < 6; q++) {
     io.in[q] = B[q];
     io.out[q] = A[q];
   }
+  io.in[6] = io.out[6] = nullptr;
   const T t1 = t0 + dt;
   return timed(c, "ez_ko", s, [&] {
     if (dg) {
@@ -333,10 +357,18 @@ int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t
   T* B[7];
   state_ptrs<T>(f, A);
   for (int q = 0; q < 7; q++) B[q] = (T*)c->scratch[q];
+  T* const Cp = (T*)c->scratch[7];
+  if (!Cp) return fail(c, COFFEE_ERR_STATE, "flat step needs the third P buffer");
 
-  // Frozen (non-periodic) guards must be present in the partner buffer.
+  // Frozen (non-periodic) guards must be present in the partner buffers.
   rc = guard_slabs<T>(c, s, B, A, false);
   if (rc) return rc;
+  {
+    T* const cd[7] = {Cp, Cp, Cp, Cp, Cp, Cp, Cp};
+    T* const cs[7] = {A[6], A[6], A[6], A[6], A[6], A[6], A[6]};
+    rc = guard_slabs<T>(c, s, cd, cs, false, 1);
+    if (rc) return rc;
+  }
 
   FlatIO<T> io{};
   for (int q = 0; q < 3; q++) {
@@ -371,12 +403,12 @@ int ez_step_impl(coffee_ctx* c, const coffee_fields* f, double time, hipStream_t
   int tx = 64, ty = 8;
   tile_shape<T>(c->variant, tx, ty);
   constexpr bool f32 = sizeof(T) == 4;
-  if (tx == 128 && ty == 4) rc = ez_launch_all<T, 128, 4>(c, k, io, A, B, time, s, star);
-  else if (tx == 128) rc = ez_launch_all<T, 128, f32 ? 8 : 4>(c, k, io, A, B, time, s, star);
-  else if (tx == 256) rc = ez_launch_all<T, f32 ? 256 : 128, 4>(c, k, io, A, B, time, s, star);
-  else if (ty == 4) rc = ez_launch_all<T, 64, 4>(c, k, io, A, B, time, s, star);
-  else if (ty == 16) rc = ez_launch_all<T, 64, 16>(c, k, io, A, B, time, s, star);
-  else rc = ez_launch_all<T, 64, 8>(c, k, io, A, B, time, s, star);
+  if (tx == 128 && ty == 4) rc = ez_launch_all<T, 128, 4>(c, k, io, A, B, Cp, time, s, star);
+  else if (tx == 128) rc = ez_launch_all<T, 128, f32 ? 8 : 4>(c, k, io, A, B, Cp, time, s, star);
+  else if (tx == 256) rc = ez_launch_all<T, f32 ? 256 : 128, 4>(c, k, io, A, B, Cp, time, s, star);
+  else if (ty == 4) rc = ez_launch_all<T, 64, 4>(c, k, io, A, B, Cp, time, s, star);
+  else if (ty == 16) rc = ez_launch_all<T, 64, 16>(c, k, io, A, B, Cp, time, s, star);
+  else rc = ez_launch_all<T, 64, 8>(c, k, io, A, B, Cp, time, s, star);
   if (rc) return rc;
   // single-GPU exchange: periodic guards of the caller's arrays
   return guard_slabs<T>(c, s, A, A, true);
@@ -496,7 +528,8 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
     return COFFEE_OK;
   }
   size_t bytes = cells(*grid) * (size_t)scalar_bytes;
-  for (int q = 0; q < 7; q++) {
+  const int nscratch = solver == COFFEE_SOLVER_EZ ? 8 : 7;
+  for (int q = 0; q < nscratch; q++) {
     hipError_t e = hipMalloc(&c->scratch[q], bytes);
     if (e != hipSuccess) {
       std::string m = std::string("hipMalloc scratch: ") + hipGetErrorString(e);
@@ -518,7 +551,7 @@ int coffee_ctx_create(int solver, int scalar_bytes, const coffee_grid* grid,
 
 int coffee_ctx_destroy(coffee_ctx* c) {
   if (!c) return COFFEE_OK;
-  for (int q = 0; q < 7; q++)
+  for (int q = 0; q < 8; q++)
     if (c->scratch[q]) (void)hipFree(c->scratch[q]);
   if (c->d_part) (void)hipFree(c->d_part);
   delete c->state;
@@ -559,9 +592,9 @@ int coffee_ez_step_noalloc(const coffee_grid* grid, const coffee_params* params,
   int rc = validate(nullptr, grid, params);
   if (rc) return rc;
   const unsigned long long per = (cells(*grid) * (unsigned long long)scalar_bytes + 255ull) & ~255ull;
-  if (7ull * per > COFFEE_NOALLOC_SCRATCH_BYTES)
+  if (8ull * per > COFFEE_NOALLOC_SCRATCH_BYTES)
     return fail(nullptr, COFFEE_ERR_SIZE,
-                "coffee_ez_step_noalloc: the partner state (7 x " + std::to_string(per) +
+                "coffee_ez_step_noalloc: the partner state (8 x " + std::to_string(per) +
                     " bytes) exceeds COFFEE_NOALLOC_SCRATCH_BYTES (" +
                     std::to_string(COFFEE_NOALLOC_SCRATCH_BYTES) + ")");
   int dev = 0;
@@ -579,7 +612,7 @@ int coffee_ez_step_noalloc(const coffee_grid* grid, const coffee_params* params,
   c.params = *params;
   fill_k<float>(*grid, *params, c.kf);
   fill_k<double>(*grid, *params, c.kd);
-  for (int q = 0; q < 7; q++) c.scratch[q] = (unsigned char*)base + q * per;
+  for (int q = 0; q < 8; q++) c.scratch[q] = (unsigned char*)base + q * per;
   hipStream_t s = (hipStream_t)stream;
   rc = scalar_bytes == 4 ? ez_step_impl<float>(&c, f, time, s) : ez_step_impl<double>(&c, f, time, s);
   if (rc) return fail(nullptr, rc, c.err);

@@ -38,7 +38,11 @@ struct coffee_ctx {
   coffee_params params{};
   coffee::FlatK<float> kf{};
   coffee::FlatK<double> kd{};
-  void* scratch[7] = {};  // ping-pong partner buffers
+  // ping-pong partner buffers of the 7 state arrays; flat solver: [7] is
+  // the third buffer of P (its five stages rotate over three buffers, so the
+  // last stage writes P into the caller's array and Kreiss-Oliger, which
+  // leaves P unchanged, does not copy it)
+  void* scratch[8] = {};
   double* d_part = nullptr;
   int part_blocks = 0;
   std::vector<double> h_part;

@@ -474,7 +474,8 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
     }
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
+  for (int q = 0; q < 6; q++) io.out[q][c] = u[q];
+  if (io.out[6]) io.out[6][c] = u[6];
 }
 
 // Per-cell end of the Kreiss-Oliger pass (field_solver_EZ.cpp:390-457 with
@@ -500,7 +501,8 @@ __device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T
     star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
   }
 #pragma unroll
-  for (int q = 0; q < 7; q++) io.out[q][c] = u[q];
+  for (int q = 0; q < 6; q++) io.out[q][c] = u[q];
+  if (io.out[6]) io.out[6][c] = u[6];
 }
 
 // Simple stage kernel: one thread per cell, stencil neighbours read straight
@@ -562,7 +564,7 @@ __global__ void __launch_bounds__(256) ez_ko_kernel(const FlatK<T> k, const Flat
     T ko = d6(f, c, ox) + d6(f, c, oy) + d6(f, c, oz);
     u[q] = fma(ko, k.KOfac, f[c]);
   }
-  u[6] = io.in[6][c];
+  u[6] = io.in[6] ? io.in[6][c] : T(0);
   T e_before = T(0);
   if (DIAG) {
     T v[6];
@@ -605,6 +607,59 @@ __global__ void __launch_bounds__(256)
     const int di = cc[0] + (cc[1] + cc[2] * k.dims[1]) * k.dims[0];
     const int si = s[0] + (s[1] + s[2] * k.dims[1]) * k.dims[0];
     for (int q = 0; q < nfields; q++) dst.p[q][di] = src.p[q][si];
+  }
+}
+
+// Periodic guard fill of every guard cell in one pass: guard cell (i, j, k)
+// takes the cell whose coordinate along each periodic axis is wrapped into the
+// interior (other axes keep theirs).  This is the result of the reference's
+// x, then y, then z slab sequence (sim_env.cpp:365-438: a corner is copied
+// along x, the copy along y, that along z, ending at the fully wrapped
+// interior cell; along a frozen axis the copies keep the frozen guard row).
+// Guard cells are enumerated without overlap: the z slabs (whole x-y
+// planes), then the y slabs of interior planes, then the x slabs of interior
+// rows.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    guard_wrap_kernel(const FlatK<T> k, const Ptr7<T> a, int nfields) {
+  const long d0 = k.dims[0], d1 = k.dims[1];
+  const int g0 = k.guard[0], g1 = k.guard[1], g2 = k.guard[2];
+  const long nzs = 2L * g2 * d0 * d1;
+  const long nys = 2L * g1 * d0 * k.n[2];
+  const long nxs = 2L * g0 * k.n[1] * k.n[2];
+  const long total = nzs + nys + nxs;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    int i, j, kk;
+    if (t < nzs) {
+      i = (int)(t % d0);
+      const long r = t / d0;
+      j = (int)(r % d1);
+      const int m = (int)(r / d1);
+      kk = m < g2 ? m : k.dims[2] - 2 * g2 + m;
+    } else if (t < nzs + nys) {
+      const long u = t - nzs;
+      i = (int)(u % d0);
+      const long r = u / d0;
+      const int m = (int)(r % (2 * g1));
+      kk = g2 + (int)(r / (2 * g1));
+      j = m < g1 ? m : k.dims[1] - 2 * g1 + m;
+    } else {
+      const long u = t - nzs - nys;
+      const int m = (int)(u % (2 * g0));
+      const long r = u / (2 * g0);
+      j = g1 + (int)(r % k.n[1]);
+      kk = g2 + (int)(r / k.n[1]);
+      i = m < g0 ? m : k.dims[0] - 2 * g0 + m;
+    }
+    int si = i, sj = j, sk = kk;
+    if (k.per[0]) si = i < g0 ? i + k.n[0] : (i >= g0 + k.n[0] ? i - k.n[0] : i);
+    if (k.per[1]) sj = j < g1 ? j + k.n[1] : (j >= g1 + k.n[1] ? j - k.n[1] : j);
+    if (k.per[2]) sk = kk < g2 ? kk + k.n[2] : (kk >= g2 + k.n[2] ? kk - k.n[2] : kk);
+    if (si == i && sj == j && sk == kk) continue;  // frozen guard cell
+    const long di = i + (j + (long)kk * d1) * d0;
+    const long src = si + (sj + (long)sk * d1) * d0;
+    for (int q = 0; q < nfields; q++) a.p[q][di] = a.p[q][src];
   }
 }
 

@@ -279,8 +279,11 @@ __global__ void __launch_bounds__(TX * TY)
 #pragma unroll
         for (int f = 0; f < NF; f++) hv[d][h][f] = io.in[f][hcol[h] + off];
   }
+  // P passes through unchanged when the caller routes it here (io.in[6]
+  // set); the step schedule leaves it out (in[6] = out[6] = null)
+  const bool has_p = io.in[6] != nullptr;
   T pc = T(0);
-  if (active) pc = io.in[6][col + (z0 + k.guard[2]) * k.sz];
+  if (active && has_p) pc = io.in[6][col + (z0 + k.guard[2]) * k.sz];
 
   for (int z = z0; z < z1; z++) {
     const int buf = (z - z0) & 1;
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(TX * TY)
           for (int f = 0; f < NF; f++) hv[HD - 1][h][f] = io.in[f][hcol[h] + off];
     }
     T pn = T(0);
-    if (active && z + 1 < z1) pn = io.in[6][c + k.sz];
+    if (active && has_p && z + 1 < z1) pn = io.in[6][c + k.sz];
     __syncthreads();
     if (active) {
       T u[7];

@@ -11,7 +11,10 @@ Pipeline, all on one GPU with the project's kernels:
      network's prediction; the spin-down luminosity at 1.5 R_LC is sampled
      every `check` steps, and each start's cost is the first step after which
      L stays within `tol` of the dipole run's final luminosity.  The field
-     error of the prediction against the relaxed state is reported too.
+     error of the prediction against the relaxed state is reported too, and
+     the solver's own residual monitor (max |div B|, max |E.B|/B^2,
+     max (E^2 - B^2)/B^2 and non-finite cells, coffee_sph_batch_residuals)
+     of both starts and of both runs at the end.
 
 The dipole start remains the fallback: a prediction is used only as initial
 data and the solver does all the evolution.
@@ -171,6 +174,13 @@ def luminosity_history(runs: Runs, check: int, total: Sequence[int]):
     return hist
 
 
+def _res_summary(r: dict) -> dict:
+    """The residual monitor's figures of one run (coffee_sph_batch_residuals):
+    max |div B|, max |E.B|/B^2, max (E^2 - B^2)/B^2 (metric norms) and the
+    number of non-finite cells."""
+    return {q: float(r[q]) for q in ("max_divB", "max_EdotB", "max_E2mB2", "nonfinite")}
+
+
 def settle_step(hist, L_ref: float, tol: float):
     """First sampled step after which L stays within tol of L_ref."""
     last_bad = 0
@@ -273,8 +283,11 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
     unpack_runs(warm, pred, cout)
     if corotate:
         corotate_runs(warm)
+    # the solver's own residual monitor on both starts and at the end
+    r_start = (dip.batch.residuals(), warm.batch.residuals())
     h_dip = luminosity_history(dip, check, steps)
     h_warm = luminosity_history(warm, check, steps)
+    r_end = (dip.batch.residuals(), warm.batch.residuals())
     # field error of the prediction vs the relaxed (dipole-run) state, in network units
     relaxed = torch.zeros_like(pred)
     pack_runs(dip, relaxed, cout, True)
@@ -299,6 +312,10 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
                     "settle_by_tol": by_tol,
                     "total_steps": steps[b], "field_err_warm": err[b][0],
                     "field_err_dipole": err[b][1],
+                    "residuals": {"start_dipole": _res_summary(r_start[0][b]),
+                                  "start_warm": _res_summary(r_start[1][b]),
+                                  "end_dipole": _res_summary(r_end[0][b]),
+                                  "end_warm": _res_summary(r_end[1][b])},
                     "L_dipole": [round(L / L_ref, 4) for _, L in h_dip[b][::stride]],
                     "L_warm": [round(L / L_ref, 4) for _, L in h_warm[b][::stride]],
                     "L_steps": [st for st, _ in h_dip[b][::stride]]})
@@ -322,6 +339,10 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
         "train_s": t_train, "median_gain": float(np.median(gains)) if gains else None,
         "gain_by_tol": gain_by_tol,
         "n_settled_both": len(gains),
+        "median_residuals": {
+            k: {q: float(np.median([r["residuals"][k][q] for r in res]))
+                for q in ("max_divB", "max_EdotB", "max_E2mB2", "nonfinite")}
+            for k in ("start_dipole", "start_warm", "end_dipole", "end_warm")},
         "median_field_err_warm": float(np.median([r["field_err_warm"] for r in res])),
         "median_field_err_dipole": float(np.median([r["field_err_dipole"] for r in res])),
         "runs": res, "wall_s": time.perf_counter() - t_start,

@@ -143,10 +143,11 @@ int coffee_set_params(coffee_ctx *ctx, const coffee_params *params);
  * when an absorbing layer is active. */
 int coffee_ez_step(coffee_ctx *ctx, const coffee_fields *f, double time, void *stream);
 
-/* Largest partner state coffee_ez_step_noalloc accepts: 7 arrays of
+/* Largest partner state coffee_ez_step_noalloc accepts: 8 arrays (the
+ * second buffer of the 7 state arrays and a third one of P) of
  * dims[0]*dims[1]*dims[2] scalars, each rounded up to 256 bytes (80 MiB: a
- * 128^3 FP32 grid with guard 3 needs 67.4 MB; FP64 fits up to about
- * 1.49e6 cells including guards, e.g. 108^3 interior with guard 3). */
+ * 128^3 FP32 grid with guard 3 needs 77.0 MB; FP64 fits up to about
+ * 1.31e6 cells including guards, e.g. 103^3 interior with guard 3). */
 #define COFFEE_NOALLOC_SCRATCH_BYTES (80ull << 20)
 
 /* No counterpart in the reference.  coffee_ez_step without a context and
