@@ -229,6 +229,15 @@ void tile_shape(int variant, int& tx, int& ty) {
   }
 }
 
+// The FP32 default tile (tile_shape variant 0) also has a Kreiss-Oliger
+// kernel addressing the fields by 32-bit byte offsets, used by variant 0 when
+// every array is below 4 GiB; other variants (5 is the same tile) and FP64
+// use 64-bit addressing.
+template <typename T, int TX, int TY>
+constexpr bool has_o32() {
+  return sizeof(T) == 4 && TX == 128 && TY == 8;
+}
+
 template <typename T, bool F, bool L, bool D, bool S, int TX, int TY>
 void launch_stage(bool tiled, int tblocks, dim3 grid, dim3 block, hipStream_t s, const FlatK<T>& k,
                   const FlatIO<T>& io, T as, T bs, T ts, const TileMap& tm) {
@@ -249,6 +258,18 @@ void launch_stage_fl(bool dg, bool star, bool tiled, int tblocks, dim3 grid, dim
     if (star) launch_stage<T, F, L, false, true, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
     else launch_stage<T, F, L, false, false, TX, TY>(tiled, tblocks, grid, block, s, k, io, as, bs, ts, tm);
   }
+}
+
+template <typename T, int TX, int TY, bool S>
+void launch_ko_tiled(bool o32, int tblocks, hipStream_t s, const FlatK<T>& k, const FlatIO<T>& io,
+                     T t1, const TileMap& tm) {
+  if constexpr (has_o32<T, TX, TY>()) {
+    if (o32) {
+      ez_ko_tiled<T, TX, TY, S, true><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
+      return;
+    }
+  }
+  ez_ko_tiled<T, TX, TY, S, false><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
 }
 
 // Buffers of P over the five stages: A -> B -> C -> B -> C -> A (fields 0-5
@@ -284,6 +305,10 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
   const int tblocks = tm.ntx * tm.nty * tm.nzc;
   const bool tiled = c->variant != 1;
   const bool dg = c->diagnostics != 0;
+  // every field array (dims0 x dims1 x dims2 scalars) addressable by a
+  // 32-bit byte offset (Kreiss-Oliger kernel)
+  const bool o32 = c->variant == 0 &&
+                   (unsigned long long)k.dims[0] * k.dims[1] * k.dims[2] * sizeof(T) < (1ull << 32);
   const T dt = k.dt;
   const T t0 = (T)time;
   int rc = COFFEE_OK;
@@ -322,8 +347,8 @@ int ez_launch_all(coffee_ctx* c, const FlatK<T>& k, FlatIO<T> io, T* const A[7],
       if (star) ez_ko_kernel<T, true, true><<<grid, block, 0, s>>>(k, io, t1);
       else ez_ko_kernel<T, false, true><<<grid, block, 0, s>>>(k, io, t1);
     } else if (tiled) {
-      if (star) ez_ko_tiled<T, TX, TY, true><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
-      else ez_ko_tiled<T, TX, TY, false><<<tblocks, TX * TY, 0, s>>>(k, io, t1, tm);
+      if (star) launch_ko_tiled<T, TX, TY, true>(o32, tblocks, s, k, io, t1, tm);
+      else launch_ko_tiled<T, TX, TY, false>(o32, tblocks, s, k, io, t1, tm);
     } else {
       if (star) ez_ko_kernel<T, true><<<grid, block, 0, s>>>(k, io, t1);
       else ez_ko_kernel<T, false><<<grid, block, 0, s>>>(k, io, t1);

@@ -66,6 +66,29 @@ struct FlatIO {
   T* __restrict__ acc[6];
 };
 
+// Element `idx` of a field array.  O32: addressed by a 32-bit byte offset
+// from the array's base (arrays below 4 GiB; the caller checks), so a load or
+// store is one scalar base plus one vector offset shared by every field, with
+// no 64-bit address arithmetic per access; the access is cast to the global
+// address space.  Otherwise plain 64-bit indexing.  Used by the tiled
+// Kreiss-Oliger kernel of the FP32 default tile (1.74 against 1.80 ms at
+// 512^3 on one box); the stage kernels ran slower with it (3.46 against 3.37
+// ms) and FP64 the same, so they index plainly.
+template <typename T, bool O32>
+__device__ __forceinline__ T ldf(const T* base, int idx) {
+  if (O32)
+    return *(const __attribute__((address_space(1))) T*)((const char*)base +
+                                                          (unsigned)idx * (unsigned)sizeof(T));
+  return base[idx];
+}
+template <typename T, bool O32>
+__device__ __forceinline__ void stf(T* base, int idx, T v) {
+  if (O32)
+    *(__attribute__((address_space(1))) T*)((char*)base + (unsigned)idx * (unsigned)sizeof(T)) = v;
+  else
+    base[idx] = v;
+}
+
 __device__ __forceinline__ int wrap_idx(int c, int g, int n, int per) {
   if (per) {
     if (c < g) c += n;
@@ -483,7 +506,7 @@ __device__ __forceinline__ void stage_cell(const FlatK<T>& k, const FlatIO<T>& i
 // f + (KOeps/64) sum_axes delta^6 f, u[6] = P (untouched on the CPU path).
 // DIAG: e_before = |E|^2 + |B|^2 before the KO update; dU_KO = the change
 // (field_solver_EZ.cpp:447-453), then the projection's accounting.
-template <typename T, bool STAR, bool DIAG = false>
+template <typename T, bool STAR, bool DIAG = false, bool O32 = false>
 __device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T>& io, int i,
                                                int j, int kk, int c, T (&u)[7], T t,
                                                T e_before = T(0)) {
@@ -501,8 +524,8 @@ __device__ __forceinline__ void ko_cell_finish(const FlatK<T>& k, const FlatIO<T
     star_boundary(k, x, y, z, t, u[0], u[1], u[2], u[3], u[4], u[5]);
   }
 #pragma unroll
-  for (int q = 0; q < 6; q++) io.out[q][c] = u[q];
-  if (io.out[6]) io.out[6][c] = u[6];
+  for (int q = 0; q < 6; q++) stf<T, O32>(io.out[q], c, u[q]);
+  if (io.out[6]) stf<T, O32>(io.out[6], c, u[6]);
 }
 
 // Simple stage kernel: one thread per cell, stencil neighbours read straight

@@ -15,6 +15,9 @@
 // workgroups are dispatched round-robin over the 8 XCDs, and the remap gives
 // each XCD a contiguous range of tiles so that halo rows hit its own L2.
 //
+// O32 (Kreiss-Oliger kernel): fields addressed by 32-bit byte offsets
+// (arrays below 4 GiB, chosen by the host), see ldf / stf.
+//
 // Partial tiles (n not a multiple of the tile) are handled by clamping loads
 // of inactive lanes into the array and masking their stores, so every grid
 // the simple kernels accept runs here and gives bitwise the same results
@@ -219,7 +222,7 @@ __global__ void __launch_bounds__(TX * TY)
   }
 }
 
-template <typename T, int TX, int TY, bool STAR>
+template <typename T, int TX, int TY, bool STAR, bool O32>
 __global__ void __launch_bounds__(TX * TY)
     ez_ko_tiled(const FlatK<T> k, const FlatIO<T> io, const T tko, const TileMap tm) {
   constexpr int R = 3, LX = TX + 2 * R, LY = TY + 2 * R, NF = 6, NT = TX * TY;
@@ -265,7 +268,7 @@ __global__ void __launch_bounds__(TX * TY)
   for (int d = 0; d < Q + LA - 1; d++) {
     const int off = amap(z0 - R + d, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
-    for (int f = 0; f < NF; f++) q[d][f] = io.in[f][col + off];
+    for (int f = 0; f < NF; f++) q[d][f] = ldf<T, O32>(io.in[f], col + off);
   }
   // halo of planes z .. z+HD-1, fetched with the centre columns
   constexpr int HD = deep_halo<T, STAR, HPT, NT>() ? R + LA : 1;
@@ -277,20 +280,20 @@ __global__ void __launch_bounds__(TX * TY)
     for (int h = 0; h < HPT; h++)
       if (hok[h])
 #pragma unroll
-        for (int f = 0; f < NF; f++) hv[d][h][f] = io.in[f][hcol[h] + off];
+        for (int f = 0; f < NF; f++) hv[d][h][f] = ldf<T, O32>(io.in[f], hcol[h] + off);
   }
   // P passes through unchanged when the caller routes it here (io.in[6]
   // set); the step schedule leaves it out (in[6] = out[6] = null)
   const bool has_p = io.in[6] != nullptr;
   T pc = T(0);
-  if (active && has_p) pc = io.in[6][col + (z0 + k.guard[2]) * k.sz];
+  if (active && has_p) pc = ldf<T, O32>(io.in[6], col + (z0 + k.guard[2]) * k.sz);
 
   for (int z = z0; z < z1; z++) {
     const int buf = (z - z0) & 1;
     {
       const int off = amap(z + R + LA, k.n[2], k.guard[2], k.per[2]) * k.sz;
 #pragma unroll
-      for (int f = 0; f < NF; f++) q[Q + LA - 1][f] = io.in[f][col + off];
+      for (int f = 0; f < NF; f++) q[Q + LA - 1][f] = ldf<T, O32>(io.in[f], col + off);
     }
     const int az = z + k.guard[2];
     const int c = col + az * k.sz;
@@ -313,10 +316,10 @@ __global__ void __launch_bounds__(TX * TY)
       for (int h = 0; h < HPT; h++)
         if (hok[h])
 #pragma unroll
-          for (int f = 0; f < NF; f++) hv[HD - 1][h][f] = io.in[f][hcol[h] + off];
+          for (int f = 0; f < NF; f++) hv[HD - 1][h][f] = ldf<T, O32>(io.in[f], hcol[h] + off);
     }
     T pn = T(0);
-    if (active && has_p && z + 1 < z1) pn = io.in[6][c + k.sz];
+    if (active && has_p && z + 1 < z1) pn = ldf<T, O32>(io.in[6], c + k.sz);
     __syncthreads();
     if (active) {
       T u[7];
@@ -329,7 +332,7 @@ __global__ void __launch_bounds__(TX * TY)
         u[f] = fma(kx + ky + kz, k.KOfac, q[R][f]);
       }
       u[6] = pc;
-      ko_cell_finish<T, STAR>(k, io, ai, aj, az, c, u, tko);
+      ko_cell_finish<T, STAR, false, O32>(k, io, ai, aj, az, c, u, tko);
     }
 #pragma unroll
     for (int d = 0; d < Q + LA - 1; d++)

@@ -203,6 +203,8 @@ int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
  * Scalar type (FP32 128 x 8, FP64 64 x 8), 1 = one thread per cell, 2/3/4 =
  * tiled 64 wide with 4/8/16 rows per tile (16 in FP32, 8 in FP64), 5/6 = tiled
  * 128/256 wide (FP32 128 x 8 and 256 x 4; FP64 128 x 4 for both), 7 = 128 x 4;
+ * variant 0 in FP32 runs Kreiss-Oliger with 32-bit field offsets when every
+ * array is below 4 GiB, variant 5 is the same tile with 64-bit addressing;
  * the GR solver runs its tiled schedule for every variant but 1 (1 = point
  * sweep + stage sweep), reading the halo points' metric in stages 2-5 from a
  * table it fills once, except variant 2 (every metric point computed on the

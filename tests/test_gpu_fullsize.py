@@ -6,8 +6,8 @@ hold bitwise at any size:
 
 - schedule invariance: the tiled kernels (the ones the benchmarks time) equal
   the simple one-thread-per-cell kernels, which the oracle tests pin at small
-  sizes, bitwise (flat 512^3 Alfven collision FP32, Kerr-Schild Wald 384^3
-  FP32);
+  sizes, bitwise (flat 512^3 Alfven collision FP32 and FP64, Kerr-Schild Wald
+  384^3 FP32);
 - shift equivariance of the periodic flat box: a state shifted by a whole
   number of cells steps to the shifted result, bitwise, so every tile, XCD
   and wavefront seam sees other cells than in the unshifted run.  The
@@ -50,11 +50,11 @@ def _release():
     torch.cuda.empty_cache()
 
 
-def _flat_run(sp, fields, variant, shift=None):
+def _flat_run(sp, fields, variant, shift=None, dtype=np.float32, steps=FLAT_STEPS):
     """Steps the flat solver from `fields` (guards refilled periodically, the
     interior optionally rolled by `shift` = (dz, dy, dx) cells first) and
     returns device copies of the 7 state arrays."""
-    env = SimEnvironment(sp, np.float32, "cuda:0")
+    env = SimEnvironment(sp, dtype, "cuda:0")
     data = SimData(env)
     _load(data, fields)
     solver = FieldSolverEZ(data, env)
@@ -65,7 +65,7 @@ def _flat_run(sp, fields, variant, shift=None):
             a.dev[it] = torch.roll(a.dev[it], shifts=shift, dims=(0, 1, 2))
     solver.fill_guards()
     t = 0.0
-    for _ in range(FLAT_STEPS):
+    for _ in range(steps):
         solver.evolve_fields(t)
         t += sp.dt
     torch.cuda.synchronize()
@@ -103,6 +103,26 @@ def test_flat_512_schedules_and_shift_bitwise():
     for f, (a, b) in enumerate(zip(base, shifted)):
         want = torch.roll(a[it], shifts=shift, dims=(0, 1, 2))
         assert torch.equal(b[it], want), f"field {f}: shifted run is not the shifted result"
+
+
+def test_flat_512_fp64_schedules_bitwise():
+    """The FP64 build at the benchmark size: the tiled kernels (default 64 x 8
+    tile, fields addressed by 32-bit byte offsets) equal the simple
+    one-thread-per-cell kernels (64-bit indexing) bitwise after 2 steps."""
+    sp = bench.alfven_params(FLAT_N, np.float64)
+    grid = SimEnvironment(sp, np.float64, "cuda:0").grid()
+    fields, _ = ic.alfven_packets_seeded(grid, 0, dtype=np.float64)
+    print(f"flat {FLAT_N}^3 fp64 alfven, 2 steps: tiled (default)", flush=True)
+    tiled, _ = _flat_run(sp, fields, 0, dtype=np.float64, steps=2)
+    assert all(bool(torch.isfinite(a).all()) for a in tiled)
+    print("flat fp64: simple one-thread-per-cell schedule", flush=True)
+    simple, _ = _flat_run(sp, fields, 1, dtype=np.float64, steps=2)
+    for f, (a, b) in enumerate(zip(tiled, simple)):
+        assert torch.equal(a, b), f"field {f}: tiled and simple schedules differ"
+    start = torch.from_numpy(np.ascontiguousarray(fields["By"])).to("cuda:0")
+    assert not torch.equal(tiled[4], start)
+    del simple, tiled, start, fields
+    _release()
 
 
 def _gr_run(sp, fields, variant):
