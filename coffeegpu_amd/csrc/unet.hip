@@ -234,6 +234,158 @@ __global__ void __launch_bounds__(256) conv3x3_k32(const ConvArgs a) {
   }
 }
 
+// The same convolution with the input tile in LDS, for Cin % 32 == 0,
+// Cout % 64 == 0 and W % 16 == 0.  A workgroup (4 wavefronts) computes a tile
+// of TR image rows x 16 columns of one image x 64 output channels.  For each
+// chunk of 32 input channels it stages the (TR + 2) x 18 input pixels (zero
+// outside the image) and the 9 x 64 x 32 weights in LDS once, and all nine taps
+// read their A fragments (one 16-pixel row segment, shifted by the tap) and B
+// fragments from there, so every input value crosses the memory hierarchy
+// once per chunk instead of once per tap.  The next chunk's global loads are
+// issued before the current chunk's MFMAs, into registers; NBUF = 2
+// double-buffers the LDS, NBUF = 1 stores after a barrier (half the LDS, so two
+// workgroups share a CU).  LDS rows
+// are 40 BF16 wide (32 channels + 8 padding), so the 16-byte fragment reads of
+// consecutive lanes fall in distinct banks.  Wavefront w owns rows
+// w*TR/4 .. (w+1)*TR/4 - 1: TR/4 M fragments x 4 N fragments of
+// v_mfma_f32_16x16x32_bf16.  The epilogue goes through LDS, so every output
+// pixel's 64 channels are written as 16-byte stores.
+template <int TR, int NBUF>
+__global__ void __launch_bounds__(256) conv3x3_tile(const ConvArgs a) {
+  constexpr int TW = 16, CP = 40, IH = TR + 2, IW = TW + 2, NIN = IH * IW;
+  constexpr int MB = TR / 4, NB = 4;
+  constexpr int IN_E = NIN * CP, W_E = 9 * 64 * CP, BUF_E = IN_E + W_E;
+  constexpr int IN_LD = (NIN * 4 + 255) / 256;  // 16-byte input loads per thread
+  constexpr int W_LD = 9;                       // 9 x 64 x 4 16-byte weight loads / 256
+  constexpr int EP = 68;                        // epilogue row pitch (floats)
+  static_assert(TR % 4 == 0, "rows per wavefront");
+  static_assert(TR * TW * EP * 4 <= NBUF * BUF_E * 2, "epilogue fits in the buffers");
+  __shared__ __attribute__((aligned(16))) bf16 lds[NBUF * BUF_E];
+
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tiles_x = a.W / TW, tiles_y = (a.H + TR - 1) / TR;
+  const int tile = blockIdx.x;
+  const int img = tile / (tiles_x * tiles_y);
+  const int rest = tile - img * tiles_x * tiles_y;
+  const int y0 = (rest / tiles_x) * TR, x0 = (rest % tiles_x) * TW;
+  const int co0 = blockIdx.y * 64;
+  const int nch = a.Cin / 32;
+
+  s8 rin[IN_LD], rw[W_LD];
+  auto load_chunk = [&](int ch) __attribute__((always_inline)) {
+    const int c0 = ch * 32;
+#pragma unroll
+    for (int u = 0; u < IN_LD; u++) {
+      const int idx = t + 256 * u;
+      rin[u] = s8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (idx < NIN * 4) {
+        const int pix = idx >> 2, part = idx & 3;
+        const int yy = y0 - 1 + pix / IW, xx = x0 - 1 + pix % IW;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+          rin[u] = *(const s8*)(a.in + ((long)(img * a.H + yy) * a.W + xx) * a.Cin + c0 + 8 * part);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < W_LD; u++) {
+      const int idx = t + 256 * u;  // < 9 * 64 * 4
+      const int row = idx >> 2, part = idx & 3;  // row = tap * 64 + co
+      const int tap = row >> 6, co = row & 63;
+      rw[u] = *(const s8*)(a.w + ((long)tap * a.Cout + co0 + co) * a.Cin + c0 + 8 * part);
+    }
+  };
+  auto store_chunk = [&](bf16* buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < IN_LD; u++) {
+      const int idx = t + 256 * u;
+      if (idx < NIN * 4) *(s8*)&buf[(idx >> 2) * CP + 8 * (idx & 3)] = rin[u];
+    }
+#pragma unroll
+    for (int u = 0; u < W_LD; u++) {
+      const int idx = t + 256 * u;
+      *(s8*)&buf[IN_E + (idx >> 2) * CP + 8 * (idx & 3)] = rw[u];
+    }
+  };
+
+  f4 acc[MB][NB];
+#pragma unroll
+  for (int m = 0; m < MB; m++)
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[m][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  load_chunk(0);
+  store_chunk(lds);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const bf16* buf = lds + (NBUF == 2 ? (ch & 1) * BUF_E : 0);
+    if (ch + 1 < nch) load_chunk(ch + 1);
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      if (a.flip) {
+        dy = -dy;
+        dx = -dx;
+      }
+      bf8 bfr[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+        bfr[b] = __builtin_bit_cast(bf8, *(const s8*)&buf[IN_E + (tap * 64 + b * 16 + li) * CP + 8 * lk]);
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        const int r = wave * MB + m;
+        const bf8 af = __builtin_bit_cast(
+            bf8, *(const s8*)&buf[((r + 1 + dy) * IW + (li + 1 + dx)) * CP + 8 * lk]);
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+          acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[b], acc[m][b], 0, 0, 0);
+      }
+    }
+    if (NBUF == 2) {
+      // the other buffer was last read in the previous chunk, before its barrier
+      if (ch + 1 < nch) store_chunk(lds + ((ch + 1) & 1) * BUF_E);
+    } else if (ch + 1 < nch) {
+      __syncthreads();  // every wavefront is done with this chunk
+      store_chunk(lds);
+    }
+    __syncthreads();
+  }
+
+  // epilogue: accumulators -> LDS (pixel-major, 64 channels) -> 16-byte stores
+  float* ep = (float*)lds;
+#pragma unroll
+  for (int m = 0; m < MB; m++)
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        ep[((wave * MB + m) * TW + 4 * lk + v) * EP + b * 16 + li] = acc[m][b][v];
+  __syncthreads();
+  for (int idx = t; idx < TR * TW * 8; idx += 256) {
+    const int px = idx >> 3, c8 = idx & 7;
+    const int yy = y0 + px / TW, xx = x0 + px % TW;
+    if (yy >= a.H) continue;
+    const long p = (long)(img * a.H + yy) * a.W + xx;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      float val = ep[px * EP + 8 * c8 + e] + (a.bias ? a.bias[co0 + 8 * c8 + e] : 0.f);
+      if (a.relu) val = val > 0.f ? val : 0.f;
+      v[e] = val;
+    }
+    if (a.out) {
+      s8 o;
+#pragma unroll
+      for (int e = 0; e < 8; e++) o[e] = (short)f2bf(v[e]);
+      *(s8*)(a.out + p * a.Cout + co0 + 8 * c8) = o;
+    }
+    if (a.out32) {
+      f4* dst = (f4*)(a.out32 + p * a.Cout + co0 + 8 * c8);
+      dst[0] = f4{v[0], v[1], v[2], v[3]};
+      dst[1] = f4{v[4], v[5], v[6], v[7]};
+    }
+  }
+}
+
 // One wavefront per (pixel range, tap, 16x16 tile of (co, ci)).
 __global__ void __launch_bounds__(64)
     wgrad_mfma(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
@@ -657,7 +809,23 @@ int coffee_unet_conv3x3(const void* in, const void* w, const float* bias, void* 
   const long bx = (P + 63) / 64;
   if (bx > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3: too many pixels");
   const bool aligned16 = ((uintptr_t)in % 16 == 0) && ((uintptr_t)w % 16 == 0);
-  if (cin % 32 == 0 && aligned16) {
+  const bool out_aligned = ((uintptr_t)out % 16 == 0) && ((uintptr_t)out32 % 16 == 0);
+  if (cin % 32 == 0 && cout % 64 == 0 && wd % 16 == 0 && aligned16 && out_aligned) {
+    // LDS-tiled kernel: 16-row tiles where the image has them, else 8-row
+    // single-buffered LDS (72 / 60 KB), two workgroups per CU; 16-row tiles
+    // where they still give two workgroups per CU, else 8-row tiles (measured
+    // at the U-Net shapes: 8x256x128 64->64 44 us, 8x128x64 128->128 32 us,
+    // 8x64x32 256->256 31 us; the double-buffered variant, one workgroup per
+    // CU, was 25-35 % slower on the first two)
+    const long cb = cout / 64;
+    int tr = 16;
+    if (h < 16 || (long)n * (wd / 16) * ((h + 15) / 16) * cb < 512) tr = 8;
+    const long tiles = (long)n * (wd / 16) * ((h + tr - 1) / tr);
+    if (tiles > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3: too many tiles");
+    const dim3 g((unsigned)tiles, (unsigned)cb);
+    if (tr == 16) conv3x3_tile<16, 1><<<g, 256, 0, s>>>(a);
+    else conv3x3_tile<8, 1><<<g, 256, 0, s>>>(a);
+  } else if (cin % 32 == 0 && aligned16) {
     // K = 32 kernel; two 16-pixel row blocks per wavefront when that still
     // leaves two workgroups per CU
     const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);

@@ -57,11 +57,15 @@ def test_bf16_conversion_round_to_nearest_even():
 
 # cin 16: the 16x16x16 kernel; cin % 32 == 0: the K = 32 kernel with LDS-staged
 # weights (one or two 16-pixel blocks per wavefront, 1/2/4 channel blocks,
-# partial pixel and channel tiles)
+# partial pixel and channel tiles); cin % 32 == 0, cout % 64 == 0 and
+# w % 16 == 0: the LDS-tiled kernel (8-row tiles, and 16-row tiles where they
+# make at least 512 workgroups, as in the last case; partial last tile rows,
+# several channel blocks and chunks, flipped taps)
 @pytest.mark.parametrize("cin,cout,flip,n,h,w", [
     (16, 16, 0, 2, 7, 9), (16, 48, 0, 2, 7, 9), (32, 64, 0, 2, 7, 9), (64, 32, 1, 2, 7, 9),
     (96, 80, 1, 2, 7, 9), (32, 32, 0, 2, 7, 9), (32, 64, 0, 1, 256, 256),
-    (64, 16, 1, 1, 257, 259)])
+    (64, 16, 1, 1, 257, 259), (64, 128, 1, 2, 20, 48), (32, 64, 0, 1, 10, 32),
+    (128, 64, 0, 1, 33, 16), (96, 192, 1, 1, 16, 64), (32, 64, 1, 4, 121, 256)])
 def test_conv3x3_matches_oracle(cin, cout, flip, n, h, w):
     rng = np.random.default_rng(cin + cout + flip + h)
     x = bf16_round(rng.standard_normal((n, h, w, cin)))
@@ -78,6 +82,27 @@ def test_conv3x3_matches_oracle(cin, cout, flip, n, h, w):
                                                cout, flip, 0, s()))
     got = out32.cpu().numpy().reshape(n, h, w, cout)
     assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("cin,cout,h,w", [(64, 64, 20, 32), (32, 32, 9, 7)])
+def test_conv3x3_bf16_output_is_rounded_fp32_output(cin, cout, h, w):
+    """One launch writing both outputs (bias and ReLU applied): the BF16 output
+    is the FP32 output rounded to nearest even, in both epilogues (LDS-tiled
+    kernel and K = 32 kernel)."""
+    rng = np.random.default_rng(5)
+    n = 2
+    x = bf16_round(rng.standard_normal((n, h, w, cin)))
+    W = bf16_round(rng.standard_normal((9, cout, cin)) * 0.2)
+    b = rng.standard_normal(cout).astype(np.float32)
+    xd, wd = dev_bf16(x), dev_bf16(W)
+    bd = torch.from_numpy(b).to("cuda:0")
+    out32 = torch.empty(n * h * w * cout, dtype=torch.float32, device="cuda:0")
+    out = torch.empty(n * h * w * cout, dtype=torch.bfloat16, device="cuda:0")
+    _lib.check(_lib.load().coffee_unet_conv3x3(p(xd), p(wd), p(bd), p(out), p(out32), n, h, w,
+                                               cin, cout, 0, 1, s()))
+    torch.cuda.synchronize()
+    assert float(out32.min()) >= 0.0  # ReLU
+    assert torch.equal(out32.to(torch.bfloat16), out)
 
 
 @pytest.mark.parametrize("n,h,w,cin,cout,nsplit", [(2, 6, 10, 32, 16, 3), (1, 40, 52, 64, 48, 7),
