@@ -514,6 +514,131 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Weight gradient with spatial tiles in LDS, for Cin % 64 == 0, Cout % 64 == 0
+// and W % 16 == 0: a workgroup computes the 9 x 64 (co) x 64 (ci) block of
+// dW[tap][co][ci] = sum_p dz[p][co] x[p + tap][ci] over a range of tiles of
+// TR image rows x 16 columns.  Per tile it stages dz (TR x 16 pixels x 64 co)
+// and the input with its one-pixel halo, both transposed to channel-major
+// rows, the input in three copies shifted by dx = -1, 0, 1 so that every
+// tap's B fragment (8 consecutive pixels of one row) is one aligned 16-byte
+// read; every staged value then serves all nine taps.  Wavefront w owns input
+// channels ci0 + 16 w .. + 15 against the 64 output channels: per 32-pixel
+// step it reads 4 A fragments (shared by the taps) and 9 B fragments for 36
+// MFMAs (v_mfma_f32_16x16x32_bf16).  The next tile's global loads are in
+// flight during the current tile's MFMAs.  Pixels beyond the image are zero.
+// Partial blocks per range go to part[range][tap][co][ci], summed by
+// wgrad_reduce in range order.
+template <int TR>
+__global__ void __launch_bounds__(256)
+    wgrad_tile(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
+               int N, int H, int W, int Cin, int Cout, int tiles_per_range, int ntiles) {
+  constexpr int TW = 16, TP = TR * TW, DZP = TP + 8;  // dz rows: TP pixels + 8 padding
+  constexpr int XR = TR + 2, XP = XR * TW + 8;         // shifted input rows per channel
+  constexpr int NDZ = TP * 8 / 256;                    // 16-byte dz loads per thread
+  constexpr int NXP = XR * (TW + 2), NX = (NXP * 8 + 255) / 256;
+  static_assert(TP % 32 == 0, "whole 32-pixel steps per tile");
+  __shared__ __attribute__((aligned(16))) bf16 dzt[64 * DZP];
+  __shared__ __attribute__((aligned(16))) bf16 xs[3][64 * XP];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int range = blockIdx.x;
+  const int ncib = Cin / 64;
+  const int co0 = (blockIdx.y / ncib) * 64, ci0 = (blockIdx.y % ncib) * 64;
+  const int tiles_x = W / TW, tiles_y = (H + TR - 1) / TR;
+  const int t_begin = range * tiles_per_range;
+  const int t_end = min(t_begin + tiles_per_range, ntiles);
+
+  s8 rdz[NDZ], rx[NX];
+  auto load_tile = [&](int tile) __attribute__((always_inline)) {
+    const int img = tile / (tiles_x * tiles_y);
+    const int rest = tile - img * tiles_x * tiles_y;
+    const int y0 = (rest / tiles_x) * TR, x0 = (rest % tiles_x) * TW;
+#pragma unroll
+    for (int u = 0; u < NDZ; u++) {
+      const int idx = t + 256 * u, px = idx >> 3, c8 = idx & 7;
+      const int yy = y0 + px / TW, xx = x0 + px % TW;
+      rdz[u] = s8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (yy < H) rdz[u] = *(const s8*)(dz + ((long)(img * H + yy) * W + xx) * Cout + co0 + 8 * c8);
+    }
+#pragma unroll
+    for (int u = 0; u < NX; u++) {
+      const int idx = t + 256 * u;
+      rx[u] = s8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (idx < NXP * 8) {
+        const int pix = idx >> 3, c8 = idx & 7;
+        const int yy = y0 - 1 + pix / (TW + 2), xx = x0 - 1 + pix % (TW + 2);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+          rx[u] = *(const s8*)(x + ((long)(img * H + yy) * W + xx) * Cin + ci0 + 8 * c8);
+      }
+    }
+  };
+  auto store_tile = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < NDZ; u++) {
+      const int idx = t + 256 * u, px = idx >> 3, c8 = idx & 7;
+#pragma unroll
+      for (int e = 0; e < 8; e++) dzt[(8 * c8 + e) * DZP + px] = (bf16)rdz[u][e];
+    }
+#pragma unroll
+    for (int u = 0; u < NX; u++) {
+      const int idx = t + 256 * u;
+      if (idx < NXP * 8) {
+        const int pix = idx >> 3, c8 = idx & 7;
+        const int iy = pix / (TW + 2), ix = pix % (TW + 2);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {  // copy d holds the input shifted by dx = d - 1
+          const int c = ix - d;
+          if (c >= 0 && c < TW)
+#pragma unroll
+            for (int e = 0; e < 8; e++) xs[d][(8 * c8 + e) * XP + iy * TW + c] = (bf16)rx[u][e];
+        }
+      }
+    }
+  };
+
+  f4 acc[9][4];
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+#pragma unroll
+    for (int m = 0; m < 4; m++) acc[q][m] = f4{0.f, 0.f, 0.f, 0.f};
+  if (t_begin < t_end) load_tile(t_begin);
+  for (int tile = t_begin; tile < t_end; tile++) {
+    __syncthreads();  // every wavefront is done with the previous tile
+    store_tile();
+    __syncthreads();
+    if (tile + 1 < t_end) load_tile(tile + 1);
+#pragma unroll
+    for (int st = 0; st < TP / 32; st++) {
+      // K = pixels 32 st .. 32 st + 31 = rows 2 st, 2 st + 1; a lane's 8 are
+      // row 2 st + lk / 2, columns 8 (lk % 2) ..
+      bf8 af[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        af[m] = __builtin_bit_cast(bf8, *(const s8*)&dzt[(m * 16 + li) * DZP + 32 * st + 8 * lk]);
+      const int row = 2 * st + (lk >> 1), col = 8 * (lk & 1);
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        const int dy = q / 3 - 1, d = q % 3;
+        const bf8 bfr = __builtin_bit_cast(
+            bf8, *(const s8*)&xs[d][(wave * 16 + li) * XP + (row + dy + 1) * TW + col]);
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+          acc[q][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr, acc[q][m], 0, 0, 0);
+      }
+    }
+  }
+  // D[i = co][j = ci]: lane holds co = co0 + 16 m + 4 lk + v, ci = ci0 + 16 wave + li
+  const int ci = ci0 + wave * 16 + li;
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    float* dst = part + ((long)range * 9 + q) * Cout * Cin;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) dst[(long)(co0 + m * 16 + 4 * lk + v) * Cin + ci] = acc[q][m][v];
+  }
+}
+
 // Sum of m values v(k), k = 0 .. m-1, as 8 interleaved partial sums (k mod 8)
 // combined pairwise: a fixed order (deterministic results) whose loads do
 // not wait on each other.
@@ -860,8 +985,24 @@ int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int
   const long P = (long)n * h * wd;
   long chunk = (P + nsplit - 1) / nsplit;
   chunk = (chunk + 15) / 16 * 16;
-  const int used = (int)((P + chunk - 1) / chunk);
-  if ((uintptr_t)x % 16 == 0 && (uintptr_t)dz % 16 == 0)
+  int used = (int)((P + chunk - 1) / chunk);
+  const bool al16 = (uintptr_t)x % 16 == 0 && (uintptr_t)dz % 16 == 0;
+  if (al16 && cin % 64 == 0 && cout % 64 == 0 && wd % 16 == 0) {
+    // spatial tiles of 8 rows (4 where the image is shorter), split over at
+    // most nsplit ranges of whole tiles
+    const int tr = h >= 8 ? 8 : 4;
+    const long ntiles = (long)n * (wd / 16) * ((h + tr - 1) / tr);
+    if (ntiles > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3_wgrad: too many tiles");
+    const int per = (int)((ntiles + nsplit - 1) / nsplit);
+    used = (int)((ntiles + per - 1) / per);
+    const dim3 g((unsigned)used, (unsigned)((cout / 64) * (cin / 64)));
+    if (tr == 8)
+      wgrad_tile<8><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout,
+                                       per, (int)ntiles);
+    else
+      wgrad_tile<4><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout,
+                                       per, (int)ntiles);
+  } else if (al16)
     wgrad_k32<<<dim3(used, 9, ((cout + 63) / 64) * ((cin + 63) / 64)), 256, 0, s>>>(
         (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
   else
