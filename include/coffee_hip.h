@@ -274,7 +274,8 @@ int coffee_fill_guards_arrays(const coffee_grid *grid, const int periodic[3], vo
 
 /* Streaming copy of `bytes` (a multiple of 16, 16-byte aligned pointers)
  * from src to dst, 16 bytes per lane, `unroll` (1, 2, 4, 8) loads in flight
- * per lane, `blocks` workgroups of 256 lanes (asynchronous).  No reference
+ * per lane, `blocks` workgroups of 256 lanes, each copying one contiguous
+ * chunk with nontemporal loads and stores (asynchronous).  No reference
  * counterpart: it measures the HBM bandwidth this device attains on a plain
  * read + write stream, the ceiling the stencil kernels are compared with. */
 int coffee_stream_copy(const void *src, void *dst, unsigned long long bytes, int unroll,

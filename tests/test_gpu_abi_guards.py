@@ -130,3 +130,19 @@ def test_graph_dropped_when_tile_reset():
     got = _run_batch(graph=True, reset_tile_between=True)
     for a, r in zip(got, ref):
         assert np.array_equal(a, r)
+
+
+@pytest.mark.parametrize("words,unroll,blocks", [(1, 1, 1), (1000, 4, 7), (4097, 8, 3),
+                                                 (65536, 2, 1024), (1 << 20, 4, 16384)])
+def test_stream_copy_copies_every_word(words, unroll, blocks):
+    """The bandwidth probe's copy (one contiguous chunk per workgroup) covers
+    every 16-byte word whatever the grid, including more workgroups than
+    words and chunks that end mid-row."""
+    lib = _lib.load()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    src = torch.arange(words * 4, dtype=torch.float32, device="cuda")
+    dst = torch.full_like(src, -1.0)
+    assert lib.coffee_stream_copy(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                  words * 16, unroll, blocks, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)

@@ -105,11 +105,13 @@ def test_conv3x3_bf16_output_is_rounded_fp32_output(cin, cout, h, w):
     assert torch.equal(out32.to(torch.bfloat16), out)
 
 
-# the last three take the spatial-tile kernel (cin, cout multiples of 64,
-# w of 16): 8-row tiles with a partial last row, 4-row tiles, two channel blocks
+# the last five take the spatial-tile kernel (cin, cout multiples of 64,
+# w of 16): 8-row tiles with a partial last row, 4-row tiles, two channel
+# blocks, one range (bias by the separate passes), more ranges than tiles
 @pytest.mark.parametrize("n,h,w,cin,cout,nsplit", [(2, 6, 10, 32, 16, 3), (1, 40, 52, 64, 48, 7),
                                                     (2, 12, 16, 16, 272, 2), (2, 13, 32, 64, 64, 5),
-                                                    (1, 6, 16, 128, 64, 3), (2, 20, 48, 64, 128, 4)])
+                                                    (1, 6, 16, 128, 64, 3), (2, 20, 48, 64, 128, 4),
+                                                    (1, 16, 32, 64, 64, 1), (2, 9, 16, 128, 128, 40)])
 def test_conv3x3_wgrad_matches_oracle(n, h, w, cin, cout, nsplit):
     rng = np.random.default_rng(11 + cout)
     x = bf16_round(rng.standard_normal((n, h, w, cin)))
