@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -208,9 +209,13 @@ def _sph_batch(args, runs: int, retire_scale: float = 0.0):
     rng = np.random.default_rng(args.seed)
     shapes = [(512, 256), (1024, 512), (2048, 1024)]
     envs, datas, cells, rlcs, nsteps = [], [], [], [], []
+    layer_cells = 0  # cells beyond the absorbing layer's inner edge (Un kept)
     for r in range(runs):
         nr, nth = shapes[int(rng.integers(0, len(shapes)))]
         sp, rlc = sph_sweep_params(rng, nr, nth)
+        dx = sp.size[0] / nr
+        xh = sp.lower[0] + sp.size[0] - sp.pml[0] * dx
+        layer_cells += int(np.count_nonzero(sp.lower[0] + dx * np.arange(nr) > xh)) * nth
         env = SimEnvironment(sp, np.float32, "cuda:0")
         d = SimData(env)
         f = ic.pulsar_dipole_sph(env.grid(), sp, dtype=np.float32)
@@ -223,6 +228,7 @@ def _sph_batch(args, runs: int, retire_scale: float = 0.0):
         nsteps.append(max(1, int(math.ceil(retire_scale * 5 * 2 * math.pi * rlc / sp.dt)))
                       if retire_scale > 0 else 0)
     b = SphBatch(envs, datas, nsteps=nsteps)
+    b.layer_fraction = layer_cells / float(sum(cells))
     return b, shapes, np.array(cells), np.array(rlcs), np.array(nsteps), datas
 
 
@@ -261,6 +267,13 @@ def sph_sweep_result(args, runs: int, steps: int, warmup: int) -> dict:
     cnt, ms = timing["sph_stage"]
     bytes_per_launch = SPH_SCALARS["sph_stage"] * 4 * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+    # the first stage writes, the last reads the step-start copy Un beyond the
+    # absorbing layer's inner edge
+    f_un = b.layer_fraction
+    sph_bpc = {k: 4.0 * v for k, v in SPH_SCALARS.items()}
+    sph_bpc["sph_stage_first"] += 4.0 * 7 * f_un
+    sph_bpc["sph_stage_last"] += 4.0 * 7 * f_un
+    step_scalars = sum(SPH_SCALARS[k] * (3 if k == "sph_stage" else 1) for k in SPH_SCALARS) + 14 * f_un
     lum = b.luminosity(1.5 * rlcs)
     res = b.residuals()
     # HBM bytes per middle-stage launch, measured for one tile and batch size
@@ -279,9 +292,12 @@ def sph_sweep_result(args, runs: int, steps: int, warmup: int) -> dict:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "sph_stage",
                      "algorithmic_bytes_per_launch": bytes_per_launch},
-        "step_algorithmic_GBps": SCALARS_PER_CELL_UPDATE * 4 * value / 1e9,
-        "frac_step_616B": 616.0 * value / (HBM_PEAK_GBS * 1e9),
-        "kernels": _kernel_table(timing, SPH_SCALARS, 4, cells),
+        "un_fraction": f_un,
+        "step_algorithmic_GBps": step_scalars * 4 * value / 1e9,
+        "frac_step_616B": (616.0 + 56.0 * f_un) * value / (HBM_PEAK_GBS * 1e9),
+        "kernels": {k: {"launches": c, "avg_ms": m / c,
+                        "algorithmic_GBps": sph_bpc[k] * cells / (m / c * 1e-3) / 1e9
+                        if k in sph_bpc else None} for k, (c, m) in timing.items()},
         "luminosity_1p5RLC_median": float(np.median(lum)),
         "residuals_max_divB": max(r["max_divB"] for r in res),
         "residuals_nonfinite": sum(r["nonfinite"] for r in res),
@@ -345,6 +361,45 @@ def run_sph_sweep(args):
 
 GR_SCALARS = {"gr_stage": 28, "gr_stage_first": 21, "gr_stage_last": 21, "gr_ko": 14,
               "gr_point": 18}
+# Kerr-Schild algorithmic bytes per cell add, to the field scalars, the
+# minimal FP64 metric (H, l_x, l_y, l_z = 32 B) read by each of the 5 stages
+# and, in the absorbing layer and the excision region, the step-start copy Un
+# (7 scalars written by stage 1, read by stage 5).  The tiled kernels compute
+# the metric instead of reading it; the count is the work's, not the schedule's.
+GR_METRIC_BYTES = 32
+
+
+def gr_un_fraction(sp) -> float:
+    """Fraction of interior cells where the Kerr-Schild stages keep the
+    step-start copy Un: outside the absorbing layer's inner bounds on any axis,
+    or inside r1 = 0.7 r_H (the excision damping), at the cell positions
+    lower + i delta the kernels use."""
+    from coffeegpu_amd.initial_data import cks_metric_np
+
+    n = sp.N
+    delta = [sp.size[a] / n[a] for a in range(3)]
+    pos = [sp.lower[a] + delta[a] * np.arange(n[a]) for a in range(3)]
+    lo = [sp.lower[a] + sp.pml[a] * delta[a] for a in range(3)]
+    hi = [sp.lower[a] + sp.size[a] - sp.pml[a] * delta[a] for a in range(3)]
+    pml_any = any(v > 0 for v in sp.pml)
+    outx = (pos[0] > hi[0]) | (pos[0] < lo[0])
+    outy = (pos[1] > hi[1]) | (pos[1] < lo[1])
+    r1 = 0.7 * (1.0 + math.sqrt(1.0 - sp.a * sp.a))
+    x, y = np.meshgrid(pos[0], pos[1], indexing="xy")
+    oxy = outx[None, :] | outy[:, None]
+    count = 0
+    for zk in pos[2]:
+        outer = (oxy | ((zk > hi[2]) or (zk < lo[2]))) if pml_any else np.zeros_like(oxy)
+        r = cks_metric_np(sp.a, x, y, np.full_like(x, zk))[0]
+        count += int(np.count_nonzero(outer | (r < r1)))
+    return count / float(n[0] * n[1] * n[2])
+
+
+def gr_bytes_per_cell(esz: int, f_un: float) -> dict:
+    return {"gr_stage": 28 * esz + GR_METRIC_BYTES,
+            "gr_stage_first": 21 * esz + GR_METRIC_BYTES + 7 * esz * f_un,
+            "gr_stage_last": 21 * esz + GR_METRIC_BYTES + 7 * esz * f_un,
+            "gr_ko": 14 * esz}
 
 
 UNET_SHAPES = [(8, 128, 64, 32, 32), (8, 256, 128, 64, 64), (8, 128, 64, 128, 128),
@@ -461,8 +516,14 @@ def gr_result(args, n: int, spin: float, dtype, steps: int, warmup: int,
     value = cells / (ms_per_step * 1e-3)
     esz = np.dtype(dtype).itemsize
     cnt, ms = timing["gr_stage"]
-    bytes_per_launch = GR_SCALARS["gr_stage"] * esz * cells
+    f_un = gr_un_fraction(sp)
+    bpc = gr_bytes_per_cell(esz, f_un)
+    bytes_per_launch = bpc["gr_stage"] * cells
     achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+    fields_only = GR_SCALARS["gr_stage"] * esz * cells / (ms / cnt * 1e-3) / 1e9
+    # per cell-update: 5 stages of 28 scalars + 14 for KO, the metric read per
+    # stage, Un written and read in the layer
+    step_bytes = (5 * 28 + 14) * esz + 5 * GR_METRIC_BYTES + 14 * esz * f_un
     res = solver.residuals()
     dname = "fp32" if dtype == np.float32 else "fp64"
     traffic = load_traffic(f"gr_stage_{n}_{dname}_traffic.json")
@@ -481,8 +542,15 @@ def gr_result(args, n: int, spin: float, dtype, steps: int, warmup: int,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "gr_stage", "algorithmic_bytes_per_launch": bytes_per_launch},
-        "kernels": _kernel_table(timing, GR_SCALARS, esz, cells),
+                     "kernel": "gr_stage", "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "bytes_per_cell": bpc["gr_stage"],
+                     "frac_fields_only": fields_only / HBM_PEAK_GBS},
+        "un_fraction": f_un,
+        "step_bytes_per_cell_update": step_bytes,
+        "frac_step": value * step_bytes / (HBM_PEAK_GBS * 1e9),
+        "kernels": {k: {"launches": c, "avg_ms": m / c,
+                        "algorithmic_GBps": bpc[k] * cells / (m / c * 1e-3) / 1e9 if k in bpc else None}
+                    for k, (c, m) in timing.items()},
         "residuals_after": res,
     }
 
