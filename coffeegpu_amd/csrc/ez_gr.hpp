@@ -26,6 +26,13 @@
 
 #include "ez_flat.hpp"
 
+// The library is built with -ffp-contract=off; the Kerr-Schild code from here
+// on lets each source expression's multiply-add be fused into one FMA
+// (contraction within an expression only, decided per expression as
+// written, so every kernel that inlines a routine gets the same
+// instructions and the schedules stay bitwise identical).
+#pragma clang fp contract(on)
+
 namespace coffee {
 
 template <typename T>
@@ -65,23 +72,55 @@ struct Cks {
 };
 
 // r of metric_cks_impl.hpp:17-24 and the Kerr-Schild H, l at one point.
+// 1/b, 1/sqrt(g) and sqrt(x) in FP64 from the hardware estimates (v_rcp_f64,
+// v_rsq_f64) refined by two Newton steps each: within a few ulp of the
+// correctly rounded FP64 values, far below the FP32 rounding that the single
+// precision solver applies to every metric quantity, in about half the
+// instructions of the IEEE division and square root sequences.
+__device__ __forceinline__ double rcp_nr(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  y = fma(y, fma(-b, y, 1.0), y);
+  y = fma(y, fma(-b, y, 1.0), y);
+  return y;
+}
+__device__ __forceinline__ double rsq_nr(double g) {  // y (3 - g y^2) / 2, twice
+  const double h = 0.5 * g;
+  double y = __builtin_amdgcn_rsq(g);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return y;
+}
+__device__ __forceinline__ double sqrt_nr(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
+
+// FAST: the Newton-refined reciprocals and roots above (the FP32 solver's
+// stages and KO); otherwise IEEE FP64 (the FP64 solver, energy, residuals).
+template <bool FAST = false>
 __device__ __forceinline__ Cks cks_point(double a, double x, double y, double z) {
   const double tiny = 1e-11;
   const double R2 = x * x + y * y + z * z;
   const double w = R2 - a * a;
   const double az = a * z;
   Cks m;
-  m.r = sqrt((w + sqrt(fmax(w * w + 4.0 * (az * az), tiny))) / 2.0);
+  if constexpr (FAST)
+    m.r = sqrt_nr(0.5 * (w + sqrt_nr(fmax(w * w + 4.0 * (az * az), tiny))));
+  else
+    m.r = sqrt((w + sqrt(fmax(w * w + 4.0 * (az * az), tiny))) / 2.0);
   const double r = m.r, r2 = r * r;
   const double Q = fmax(r2 * r2 + az * az, tiny);
   const double A = fmax(a * a + r2, tiny);
   const double r3 = 2.0 * r * r2;
-  m.s = r3 / Q;
-  m.bu = r3 / fmax(r3 + r2 * r2 + az * az, tiny);
-  const double iA = 1.0 / A;
+  const double Qb = fmax(r3 + r2 * r2 + az * az, tiny);
+  if constexpr (FAST) {
+    m.s = r3 * rcp_nr(Q);
+    m.bu = r3 * rcp_nr(Qb);
+  } else {
+    m.s = r3 / Q;
+    m.bu = r3 / Qb;
+  }
+  const double iA = FAST ? rcp_nr(A) : 1.0 / A;
   m.lx = (r * x + a * y) * iA;
   m.ly = (r * y - a * x) * iA;
-  m.lz = r > 0.0 ? z / r : 0.0;
+  m.lz = r > 0.0 ? (FAST ? z * rcp_nr(r) : z / r) : 0.0;
   return m;
 }
 
@@ -91,10 +130,11 @@ struct CksT {
   double r;
 };
 
-template <typename T>
+template <typename T, bool FAST>
 __device__ __forceinline__ CksT<T> cks_scalar(const Cks& m) {
   const double g = 1.0 + m.s;
-  const double sg = sqrt(g);
+  const double ia = FAST ? rsq_nr(g) : 0.0;
+  const double sg = FAST ? g * ia : sqrt(g);
   CksT<T> t;
   t.r = m.r;
   t.s = (T)m.s;
@@ -102,9 +142,19 @@ __device__ __forceinline__ CksT<T> cks_scalar(const Cks& m) {
   t.l[1] = (T)m.ly;
   t.l[2] = (T)m.lz;
   t.sg = (T)sg;
-  t.alpha = (T)(1.0 / sg);
+  t.alpha = FAST ? (T)ia : (T)(1.0 / sg);
   t.bfac = (T)m.bu;
   return t;
+}
+
+// The metric the solver kernels use at a point: Newton-refined FP64 in the
+// FP32 solver's stages, IEEE FP64 in its KO (where the refined sequences
+// raised the register count from 118 to 188 and halved the occupancy) and
+// everywhere in the FP64 solver.  Every schedule of a kernel uses the same
+// choice, so the schedules stay bitwise identical.
+template <typename T, bool FAST = sizeof(T) == 4>
+__device__ __forceinline__ CksT<T> gr_metric(double a, double x, double y, double z) {
+  return cks_scalar<T, FAST>(cks_point<FAST>(a, x, y, z));
 }
 
 template <typename T>
@@ -265,7 +315,7 @@ __global__ void __launch_bounds__(256) gr_point_kernel(const GrK<T> k, const GrI
     const int r = c / k.dims[0];
     const int j = r % k.dims[1];
     const int kk = r / k.dims[1];
-    const CksT<T> m = cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, i), gr_pos(k, 1, j), gr_pos(k, 2, kk)));
+    const CksT<T> m = gr_metric<T>(k.a, gr_pos(k, 0, i), gr_pos(k, 1, j), gr_pos(k, 2, kk));
     const T D[3] = {io.in[0][c], io.in[1][c], io.in[2][c]};
     const T B[3] = {io.in[3][c], io.in[4][c], io.in[5][c]};
     T E[3], H[3];
@@ -377,7 +427,7 @@ __device__ __forceinline__ void gr_stage_cell(const GrK<T>& k, const GrIO<T>& io
                                               int kk, int c, T As, T Bs) {
   const int sy = k.sy, sz = k.sz;
   const double x = gr_pos(k, 0, i), y = gr_pos(k, 1, j), z = gr_pos(k, 2, kk);
-  const CksT<T> m = cks_scalar<T>(cks_point(k.a, x, y, z));
+  const CksT<T> m = gr_metric<T>(k.a, x, y, z);
   const T* const* W = (const T* const*)io.W;
   const T* P = io.in[6];
   T ddx[7], ddy[7], ddz[7];
@@ -434,7 +484,7 @@ __global__ void __launch_bounds__(256) gr_ko_kernel(const GrK<T> k, const GrIO<T
     const T t = d6(p, c, ox) + d6(p, c, oy) + d6(p, c, oz);
     u[f] = fma(t, k.KOfac, p[c]);
   }
-  const CksT<T> m = cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak)));
+  const CksT<T> m = gr_metric<T, false>(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak));
   gr_project(k, m, u);
 #pragma unroll
   for (int f = 0; f < 7; f++) io.out[f][c] = u[f];

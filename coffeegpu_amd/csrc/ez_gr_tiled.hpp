@@ -20,9 +20,9 @@
 
 namespace coffee {
 
-template <typename T>
+template <typename T, bool FAST = sizeof(T) == 4>
 __device__ __forceinline__ CksT<T> gr_metric_at(const GrK<T>& k, int ai, int aj, int ak) {
-  return cks_scalar<T>(cks_point(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak)));
+  return gr_metric<T, FAST>(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak));
 }
 
 // Metric of a point from the table io.W[0..6] (s, l_x, l_y, l_z, alpha,
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(64 * TY)
         const T kz = d6v(q[0][f], q[1][f], q[2][f], q[3][f], q[4][f], q[5][f], q[6][f]);
         u[f] = fma(kx + ky + kz, k.KOfac, q[R][f]);
       }
-      const CksT<T> m = gr_metric_at(k, ai, aj, az);
+      const CksT<T> m = gr_metric_at<T, false>(k, ai, aj, az);
       gr_project(k, m, u);
       const int c = col + az * k.sz;
 #pragma unroll
