@@ -57,6 +57,7 @@ void fill_gr(const coffee_grid& g, const coffee_params& p, GrK<T>& k) {
   k.a = p.a;
   k.dt = (T)p.dt;
   k.tau = (T)p.tau;
+  k.itau = (T)(1.0 / p.tau);
   k.KOfac = -(T)p.KOeps * (T)(-1.0 / 64.0);
   k.calc_current = p.calc_current;
   k.divB_clean = p.divB_clean;

@@ -42,7 +42,7 @@ struct GrK {
   double lower[3], delta[3];  // Grid::pos in FP64
   double a;                   // spin
   T c12[3];                   // inv_delta / 12
-  T dt, tau, KOfac;
+  T dt, tau, itau, KOfac;  // itau = 1 / tau
   int calc_current, divB_clean, clean_ep, check_egb;
   int pml_any;
   double xl[3], xh[3];  // outer absorbing-layer bounds (boundary.cu:41-52)
@@ -204,9 +204,9 @@ __device__ __forceinline__ void gr_project(const GrK<T>& k, const CksT<T>& m, T 
   const T B2raw = dot3(B, Bd);
   if (k.clean_ep) {
     const T B2 = B2raw < tiny ? tiny : B2raw;
-    const T DB = dot3(D, Bd);
+    const T f = dot3(D, Bd) / B2;  // one division for the three components
 #pragma unroll
-    for (int i = 0; i < 3; i++) D[i] = D[i] - DB * B[i] / B2;
+    for (int i = 0; i < 3; i++) D[i] = D[i] - f * B[i];
   }
   if (k.check_egb) {
     T Dd[3];
@@ -374,9 +374,10 @@ __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io
     if (B2 < tiny) B2 = tiny;
     const T Jp = dot3(Bd, rotH) - dot3(Dd, rotE);
     const T rho = divD * m.alpha;
-    J[0] = (rho * (Ec[1] * Bd[2] - Ec[2] * Bd[1]) + Jp * B[0]) / B2;
-    J[1] = (rho * (Ec[2] * Bd[0] - Ec[0] * Bd[2]) + Jp * B[1]) / B2;
-    J[2] = (rho * (Ec[0] * Bd[1] - Ec[1] * Bd[0]) + Jp * B[2]) / B2;
+    const T iB2 = T(1) / B2;  // one division for the three components
+    J[0] = (rho * (Ec[1] * Bd[2] - Ec[2] * Bd[1]) + Jp * B[0]) * iB2;
+    J[1] = (rho * (Ec[2] * Bd[0] - Ec[0] * Bd[2]) + Jp * B[1]) * iB2;
+    J[2] = (rho * (Ec[0] * Bd[1] - Ec[1] * Bd[0]) + Jp * B[2]) * iB2;
   }
   const T gP[3] = {ddx[6], ddy[6], ddz[6]};
   const T lgP = dot3(m.l, gP);
@@ -396,7 +397,7 @@ __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io
     }
     du[q] = As * du[q] + dt * (rotH[q] - J[q]);
   }
-  du[6] = As * du[6] + dt * (-m.alpha * divB + dot3(beta, gP)) - m.alpha * u[6] / k.tau;
+  du[6] = As * du[6] + dt * (-m.alpha * divB + dot3(beta, gP)) - m.alpha * u[6] * k.itau;
   if (FIRST && gr_needs_un(k, x, y, z, m.r)) {
 #pragma unroll
     for (int f = 0; f < 7; f++) io.Un[f][c] = u[f];
