@@ -536,8 +536,9 @@ def gr_result(args, n: int, spin: float, dtype, steps: int, warmup: int,
         "value": value, "unit": "cell-updates/s", "ms_per_step": ms_per_step, "steps": steps,
         "warmup": warmup, "dtype": "f32" if dtype == np.float32 else "f64",
         "config": {"workload": f"gr-wald {n}^3 box [-20,20]^3 a={sp.a}, pml 32 (sigpml 10, "
-                               "pmllen 1), horizon excision, FP64 metric on the fly (halo "
-                               "points of stages 2-5 from a precomputed table)",
+                               "pmllen 1), horizon excision, FP64 metric on the fly"
+                               + (" (halo points of stages 2-5 from a precomputed table)"
+                                  if variant == 2 else ""),
                    "cells": cells, "schedule": "tiled" if variant != 1 else "point+stage"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -566,7 +567,8 @@ def run_gr(args):
            "data": "synthetic (Wald start B = z/sqrt(gamma) + 1e-3 seeded noise)",
            "config": dict(r["config"], parallelism="single GPU"), "roofline": r["roofline"],
            "kernels": r["kernels"], "residuals_after": r["residuals_after"],
-           "cpu_baseline": None}
+           "un_fraction": r["un_fraction"], "step_bytes_per_cell_update": r["step_bytes_per_cell_update"],
+           "frac_step": r["frac_step"], "cpu_baseline": None}
     print(json.dumps(out))
 
 

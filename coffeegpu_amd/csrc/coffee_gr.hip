@@ -29,8 +29,9 @@ const double kBs[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994
 
 struct GrState : SolverState {
   void* W[12] = {};
-  // W[0..6] hold the halo metric table of the tiled schedule until the
-  // two-sweep schedule (variant 1) overwrites them
+  // allocated on first use by the schedules that need them: W[0..6] hold the
+  // halo metric table of variant 2 until the two-sweep schedule (variant 1)
+  // overwrites them
   bool mtab_valid = false;
   GrK<float> kf{};
   GrK<double> kd{};
@@ -189,12 +190,25 @@ int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
   dim3 block(64, 4, 1);
   dim3 grid((k.n[0] + 63) / 64, (k.n[1] + 3) / 4, k.n[2]);
   // variant 1: point sweep + stage sweep; others: tiled stage kernel, which
-  // in stages 2-5 reads the halo points' metric from a table (filled once;
-  // at 384^3 FP32: middle stage 2.50 -> 2.44 ms, last 2.96 -> 2.70) except
-  // in variant 2, which computes every point's metric on the fly.  Stage 1,
-  // store-heavy, computes it always (with the table: 2.64 -> 2.92 ms).
+  // computes every point's metric on the fly, except variant 2, which reads
+  // the halo points' metric in stages 2-5 from a table filled once (at 384^3
+  // FP32 with the FP32 solver's Newton-refined metric: middle stage 2.06 ms
+  // on the fly against 2.33 with the table, whose reads add 1.2 GB per
+  // launch).  Stage 1 computes it always.
   const bool tiled = c->variant != 1;
-  const bool mt = tiled && c->variant != 2;
+  const bool mt = tiled && c->variant == 2;
+  if (!tiled || mt) {
+    // the point sweep's E, H, sqrt(g) D, sqrt(g) B and the table live in
+    // scratch arrays allocated on first use
+    const size_t bytes = (size_t)k.dims[0] * k.dims[1] * k.dims[2] * sizeof(T);
+    for (int q = 0; q < 12; q++)
+      if (!S->W[q]) {
+        hipError_t e = hipMalloc(&S->W[q], bytes);
+        if (e != hipSuccess)
+          return fail(c, COFFEE_ERR_HIP, std::string("hipMalloc GR scratch: ") + hipGetErrorString(e));
+      }
+    for (int q = 0; q < 12; q++) io.W[q] = (T*)S->W[q];
+  }
   if (!tiled) S->mtab_valid = false;  // the point sweep overwrites W
   if (mt && !S->mtab_valid) {
     rc = timed(c, "gr_metric_table", s, [&] {
@@ -311,11 +325,6 @@ int gr_create(coffee_ctx* c) {
   fill_gr<double>(c->grid, c->params, S->kd);
   if (!(c->params.a > -1.0 && c->params.a < 1.0))
     return fail(c, COFFEE_ERR_INVALID, "spin a must satisfy |a| < 1");
-  const size_t bytes = (size_t)c->grid.dims[0] * c->grid.dims[1] * c->grid.dims[2] * c->scalar;
-  for (int q = 0; q < 12; q++) {
-    hipError_t e = hipMalloc(&S->W[q], bytes);
-    if (e != hipSuccess) return fail(c, COFFEE_ERR_HIP, std::string("hipMalloc GR scratch: ") + hipGetErrorString(e));
-  }
   return COFFEE_OK;
 }
 

@@ -26,13 +26,6 @@
 
 #include "ez_flat.hpp"
 
-// The library is built with -ffp-contract=off; the Kerr-Schild code from here
-// on lets each source expression's multiply-add be fused into one FMA
-// (contraction within an expression only, decided per expression as
-// written, so every kernel that inlines a routine gets the same
-// instructions and the schedules stay bitwise identical).
-#pragma clang fp contract(on)
-
 namespace coffee {
 
 template <typename T>
@@ -157,6 +150,15 @@ __device__ __forceinline__ CksT<T> gr_metric(double a, double x, double y, doubl
   return cks_scalar<T, FAST>(cks_point<FAST>(a, x, y, z));
 }
 
+// The library is built with -ffp-contract=off.  The solver arithmetic from
+// here on (not the positions and the metric above, not the energy and
+// residual reductions at the end of this file, which keep the plain
+// operation order of their host counterparts) lets each source expression's
+// multiply-add be fused into one FMA: contraction within an expression only,
+// decided per expression as written, so every kernel that inlines a routine
+// gets the same instructions and the schedules stay bitwise identical.
+#pragma clang fp contract(on)
+
 template <typename T>
 __device__ __forceinline__ void gr_lower(const CksT<T>& m, const T v[3], T out[3]) {
   const T lv = m.l[0] * v[0] + m.l[1] * v[1] + m.l[2] * v[2];
@@ -172,6 +174,7 @@ __device__ __forceinline__ T dot3(const T u[3], const T v[3]) {
 
 template <typename T>
 __device__ __forceinline__ double gr_pos(const GrK<T>& k, int ax, int i) {
+#pragma clang fp contract(off)
   return k.lower[ax] + k.delta[ax] * (double)(i - k.guard[ax]);
 }
 
@@ -491,6 +494,7 @@ __global__ void __launch_bounds__(256) gr_ko_kernel(const GrK<T> k, const GrIO<T
   for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
 }
 
+#pragma clang fp contract(off)
 // Field energy sum over the interior outside the absorbing layers:
 // sum gamma_ij f^i f^j sqrt(gamma) (no reference counterpart: the reference
 // declares field_solver_gr_EZ::total_energy but never defines it).

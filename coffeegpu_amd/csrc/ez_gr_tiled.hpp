@@ -18,6 +18,11 @@
 #include "ez_flat_tiled.hpp"
 #include "ez_gr.hpp"
 
+// Same contraction as the solver arithmetic of ez_gr.hpp (multiply-adds
+// fused within a source expression), so the tiled and the point schedules
+// compute bitwise the same values.
+#pragma clang fp contract(on)
+
 namespace coffee {
 
 template <typename T, bool FAST = sizeof(T) == 4>

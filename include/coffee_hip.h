@@ -205,10 +205,11 @@ int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
  * 128/256 wide (FP32 128 x 8 and 256 x 4; FP64 128 x 4 for both), 7 = 128 x 4;
  * variant 0 in FP32 runs Kreiss-Oliger with 32-bit field offsets when every
  * array is below 4 GiB, variant 5 is the same tile with 64-bit addressing;
- * the GR solver runs its tiled schedule for every variant but 1 (1 = point
- * sweep + stage sweep), reading the halo points' metric in stages 2-5 from a
- * table it fills once, except variant 2 (every metric point computed on the
- * fly); zchunk = z planes per workgroup of the tiled kernels, 0 = automatic.
+ * the GR solver runs its tiled schedule, every metric point computed on the
+ * fly, for every variant but 1 (point sweep + stage sweep) and 2 (the halo
+ * points' metric in stages 2-5 read from a table filled once); variants 1
+ * and 2 allocate 12 grid-sized scratch arrays on first use; zchunk = z
+ * planes per workgroup of the tiled kernels, 0 = automatic.
  * Schedules change speed only: results are bitwise identical. */
 int coffee_set_schedule(coffee_ctx *ctx, int variant, int zchunk);
 

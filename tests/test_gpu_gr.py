@@ -142,8 +142,8 @@ def test_gr_rejects_bad_spin():
 @pytest.mark.parametrize("name", ["wald05_ragged", "periodic_z"])
 def test_gr_schedules_bitwise_identical(dtype, name):
     """The tiled stage kernel (E, H on the fly) equals the point + stage
-    sweeps bitwise, with the halo metric read from the table (default) or
-    computed on the fly (variant 2), with z split into chunks and with
+    sweeps bitwise, with the halo metric computed on the fly (default) or
+    read from the table (variant 2), with z split into chunks and with
     diagnostics."""
     sp, U0 = case_gr(dtype, **GR_CASES[name])
     simple, ds = run_gpu(sp, U0, dtype, 3, diag=True, variant=1)
