@@ -195,7 +195,10 @@ int gr_step_impl(coffee_ctx* c, const coffee_fields* f, hipStream_t s) {
   // FP32 with the FP32 solver's Newton-refined metric: middle stage 2.06 ms
   // on the fly against 2.33 with the table, whose reads add 1.2 GB per
   // launch).  Stage 1 computes it always.
-  const bool tiled = c->variant != 1;
+  // the tiled stage kernel addresses the fields by 32-bit byte offsets:
+  // arrays of 4 GiB or more take the point schedule
+  const bool fits32 = (size_t)k.dims[0] * k.dims[1] * k.dims[2] * sizeof(T) < (1ull << 32);
+  const bool tiled = c->variant != 1 && fits32;
   const bool mt = tiled && c->variant == 2;
   if (!tiled || mt) {
     // the point sweep's E, H, sqrt(g) D, sqrt(g) B and the table live in

@@ -351,7 +351,7 @@ constexpr int kGrDz[6] = {0, 1, 3, 4, 8, 11};
 // .cu:188-373, rk_step2 .cu:375-438, projections .cu:440-535, absorbing in
 // the last stage).  u: D, B, P at the cell (in), dprev: dU before the stage,
 // Ec: E_i at the cell, un: the cell's Un (last stage, where gr_needs_un).
-template <typename T, bool FIRST, bool LAST, bool DIAG>
+template <typename T, bool FIRST, bool LAST, bool DIAG, bool O32 = false>
 __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io, int c, double x,
                                               double y, double z, const CksT<T>& m,
                                               const T ddx[7], const T ddy[7], const T ddz[7],
@@ -403,17 +403,17 @@ __device__ __forceinline__ void gr_stage_core(const GrK<T>& k, const GrIO<T>& io
   du[6] = As * du[6] + dt * (-m.alpha * divB + dot3(beta, gP)) - m.alpha * u[6] * k.itau;
   if (FIRST && gr_needs_un(k, x, y, z, m.r)) {
 #pragma unroll
-    for (int f = 0; f < 7; f++) io.Un[f][c] = u[f];
+    for (int f = 0; f < 7; f++) stf<T, O32>(io.Un[f], c, u[f]);
   }
 #pragma unroll
   for (int f = 0; f < 7; f++) u[f] = u[f] + Bs * du[f];
   gr_project(k, m, u);
   if (LAST) gr_absorb(k, un, x, y, z, m.r, u);
 #pragma unroll
-  for (int f = 0; f < 7; f++) io.out[f][c] = u[f];
+  for (int f = 0; f < 7; f++) stf<T, O32>(io.out[f], c, u[f]);
   if (!LAST) {
 #pragma unroll
-    for (int f = 0; f < 7; f++) io.dU[f][c] = du[f];
+    for (int f = 0; f < 7; f++) stf<T, O32>(io.dU[f], c, du[f]);
   }
   if (DIAG) {
     io.J[0][c] = J[0];

@@ -208,8 +208,9 @@ int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
  * the GR solver runs its tiled schedule, every metric point computed on the
  * fly, for every variant but 1 (point sweep + stage sweep) and 2 (the halo
  * points' metric in stages 2-5 read from a table filled once); variants 1
- * and 2 allocate 12 grid-sized scratch arrays on first use; zchunk = z
- * planes per workgroup of the tiled kernels, 0 = automatic.
+ * and 2 allocate 12 grid-sized scratch arrays on first use, and GR arrays of
+ * 4 GiB or more always take the point schedule; zchunk = z planes per
+ * workgroup of the tiled kernels, 0 = automatic.
  * Schedules change speed only: results are bitwise identical. */
 int coffee_set_schedule(coffee_ctx *ctx, int variant, int zchunk);
 
