@@ -55,3 +55,29 @@ def test_sph_sweep_parameters():
         assert sp.pml[0] == math.ceil((math.log(60.0) - math.log(48.0)) / dx)
         assert math.isclose(sp.dt, 0.4 * min(dx, dth))
         assert 3.0 <= rlc <= 20.0 and math.isclose(sp.omega, 1.0 / rlc)
+
+
+def test_gr_un_fraction_matches_cell_by_cell_count():
+    """The share of cells where the Kerr-Schild stages keep Un (outside the
+    absorbing layer's inner bounds on any axis, or r < 0.7 r_H), vectorised,
+    against a plain loop over the cells of a small grid with the same
+    positions lower + i delta; and the 384^3 benchmark's 42 %."""
+    sp = bench.gr_wald_params(24, 0.9)
+    sp.pml = [4, 4, 4]
+    n, d = 24, 40.0 / 24
+    lo, hi = -20.0 + 4 * d, 20.0 - 4 * d
+    a = 0.9
+    r1 = 0.7 * (1.0 + math.sqrt(1.0 - a * a))
+    count = 0
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                x, y, z = -20.0 + d * i, -20.0 + d * j, -20.0 + d * k
+                w = x * x + y * y + z * z - a * a
+                r = math.sqrt((w + math.sqrt(max(w * w + 4 * (a * z) ** 2, 1e-11))) / 2)
+                outer = any(v > hi or v < lo for v in (x, y, z))
+                count += outer or r < r1
+    assert bench.gr_un_fraction(sp) == count / n ** 3
+    sp = bench.gr_wald_params(192, 0.9)
+    sp.pml = [16, 16, 16]  # the 384^3 layer's thickness in box units
+    assert 0.41 < bench.gr_un_fraction(sp) < 0.43
