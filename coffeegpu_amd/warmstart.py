@@ -155,7 +155,7 @@ def relax(runs: Runs, chunk: int = 500, log=None):
             log(f"relax {done}/{target}")
 
 
-def luminosity_history(runs: Runs, check: int, total: Sequence[int]):
+def luminosity_history(runs: Runs, check: int, total: Sequence[int], log=None):
     """Step every run to total[b] while sampling L(1.5 R_LC) every `check`
     steps.  Returns per-run lists of (step, L)."""
     torch = _torch()
@@ -163,6 +163,7 @@ def luminosity_history(runs: Runs, check: int, total: Sequence[int]):
     radii = 1.5 * np.asarray(runs.rlcs)
     horizon = max(total)
     step = 0
+    n = 0
     while step < horizon:
         runs.batch.step(check)
         step += check
@@ -170,6 +171,9 @@ def luminosity_history(runs: Runs, check: int, total: Sequence[int]):
         for b in range(len(runs.rlcs)):
             if step <= total[b]:
                 hist[b].append((step, float(L[b])))
+        n += 1
+        if log and n % 50 == 0:
+            log(f"evaluate {step}/{horizon}")
     torch.cuda.synchronize()
     return hist
 
@@ -285,8 +289,8 @@ def experiment(nr=256, nth=128, n_train=128, n_test=16, periods=3.0, rlc_lo=3.0,
         corotate_runs(warm)
     # the solver's own residual monitor on both starts and at the end
     r_start = (dip.batch.residuals(), warm.batch.residuals())
-    h_dip = luminosity_history(dip, check, steps)
-    h_warm = luminosity_history(warm, check, steps)
+    h_dip = luminosity_history(dip, check, steps, log=log)
+    h_warm = luminosity_history(warm, check, steps, log=log)
     r_end = (dip.batch.residuals(), warm.batch.residuals())
     # field error of the prediction vs the relaxed (dipole-run) state, in network units
     relaxed = torch.zeros_like(pred)
