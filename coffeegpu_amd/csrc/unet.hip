@@ -234,6 +234,15 @@ __global__ void __launch_bounds__(256) conv3x3_k32(const ConvArgs a) {
   }
 }
 
+// 16-byte chunk swizzle of the 64-byte (32-channel) LDS rows of conv3x3_tile:
+// chunk c of row k lives at c ^ csw(k).  On gfx950 a ds_read_b128 whose 64
+// lanes read chunk lane / 16 of rows base + lane % 16 then runs as fast as a
+// contiguous 1 KiB read, for every base (measured); unswizzled 64-byte rows,
+// rows padded to 80 bytes and the swizzles (k >> 2) & 3 or k & 3 took 1.35x
+// as long, and the forward convolution counted a third of its LDS cycles as
+// bank conflicts with the padded rows.
+__device__ __forceinline__ int csw(int k) { return (k >> 1) & 3; }
+
 // The same convolution with the input tile in LDS, for Cin % 32 == 0,
 // Cout % 64 == 0 and W % 16 == 0.  A workgroup (4 wavefronts) computes a tile
 // of TR image rows x 16 columns of one image x 64 output channels.  For each
@@ -245,22 +254,23 @@ __global__ void __launch_bounds__(256) conv3x3_k32(const ConvArgs a) {
 // issued before the current chunk's MFMAs, into registers; NBUF = 2
 // double-buffers the LDS, NBUF = 1 stores after a barrier (half the LDS, so two
 // workgroups share a CU).  LDS rows
-// are 40 BF16 wide (32 channels + 8 padding), so the 16-byte fragment reads of
-// consecutive lanes fall in distinct banks.  Wavefront w owns rows
+// are the 32 channels (64 bytes) with their 16-byte chunks swizzled by csw,
+// so the 16-byte fragment reads of consecutive lanes fall in distinct banks.
+// Wavefront w owns rows
 // w*TR/4 .. (w+1)*TR/4 - 1: TR/4 M fragments x 4 N fragments of
 // v_mfma_f32_16x16x32_bf16.  The epilogue goes through LDS, so every output
 // pixel's 64 channels are written as 16-byte stores.
 template <int TR, int NBUF>
 __global__ void __launch_bounds__(256) conv3x3_tile(const ConvArgs a) {
-  constexpr int TW = 16, CP = 40, IH = TR + 2, IW = TW + 2, NIN = IH * IW;
+  constexpr int TW = 16, CP = 32, IH = TR + 2, IW = TW + 2, NIN = IH * IW;
   constexpr int MB = TR / 4, NB = 4;
   constexpr int IN_E = NIN * CP, W_E = 9 * 64 * CP, BUF_E = IN_E + W_E;
   constexpr int IN_LD = (NIN * 4 + 255) / 256;  // 16-byte input loads per thread
   constexpr int W_LD = 9;                       // 9 x 64 x 4 16-byte weight loads / 256
   constexpr int EP = 68;                        // epilogue row pitch (floats)
+  constexpr int LDS_E = NBUF * BUF_E > TR * TW * EP * 2 ? NBUF * BUF_E : TR * TW * EP * 2;
   static_assert(TR % 4 == 0, "rows per wavefront");
-  static_assert(TR * TW * EP * 4 <= NBUF * BUF_E * 2, "epilogue fits in the buffers");
-  __shared__ __attribute__((aligned(16))) bf16 lds[NBUF * BUF_E];
+  __shared__ __attribute__((aligned(16))) bf16 lds[LDS_E];
 
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int li = lane & 15, lk = lane >> 4;
@@ -298,12 +308,12 @@ __global__ void __launch_bounds__(256) conv3x3_tile(const ConvArgs a) {
 #pragma unroll
     for (int u = 0; u < IN_LD; u++) {
       const int idx = t + 256 * u;
-      if (idx < NIN * 4) *(s8*)&buf[(idx >> 2) * CP + 8 * (idx & 3)] = rin[u];
+      if (idx < NIN * 4) *(s8*)&buf[(idx >> 2) * CP + 8 * ((idx & 3) ^ csw(idx >> 2))] = rin[u];
     }
 #pragma unroll
     for (int u = 0; u < W_LD; u++) {
       const int idx = t + 256 * u;
-      *(s8*)&buf[IN_E + (idx >> 2) * CP + 8 * (idx & 3)] = rw[u];
+      *(s8*)&buf[IN_E + (idx >> 2) * CP + 8 * ((idx & 3) ^ csw(idx >> 2))] = rw[u];
     }
   };
 
@@ -328,13 +338,14 @@ __global__ void __launch_bounds__(256) conv3x3_tile(const ConvArgs a) {
       }
       bf8 bfr[NB];
 #pragma unroll
-      for (int b = 0; b < NB; b++)
-        bfr[b] = __builtin_bit_cast(bf8, *(const s8*)&buf[IN_E + (tap * 64 + b * 16 + li) * CP + 8 * lk]);
+      for (int b = 0; b < NB; b++)  // csw(tap * 64 + b * 16 + li) = csw(li)
+        bfr[b] = __builtin_bit_cast(
+            bf8, *(const s8*)&buf[IN_E + (tap * 64 + b * 16 + li) * CP + 8 * (lk ^ csw(li))]);
 #pragma unroll
       for (int m = 0; m < MB; m++) {
         const int r = wave * MB + m;
-        const bf8 af = __builtin_bit_cast(
-            bf8, *(const s8*)&buf[((r + 1 + dy) * IW + (li + 1 + dx)) * CP + 8 * lk]);
+        const int row = (r + 1 + dy) * IW + (li + 1 + dx);
+        const bf8 af = __builtin_bit_cast(bf8, *(const s8*)&buf[row * CP + 8 * (lk ^ csw(row))]);
 #pragma unroll
         for (int b = 0; b < NB; b++)
           acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[b], acc[m][b], 0, 0, 0);
