@@ -105,9 +105,14 @@ __device__ __forceinline__ void gr_point_z(const CksT<T>& m, const T u[7], T v[7
 
 // MT: the halo points' metric is read from the table io.W[0..6] (one plane
 // ahead, with their state) instead of being computed in FP64 every plane.
-template <typename T, int TY, bool FIRST, bool LAST, bool DIAG, bool MT>
+// HD: planes of look-ahead of the x-y halo loads (1 with the table).  With
+// HD = 3 the halo of plane z + 3 is fetched in the iteration that fetches the
+// centre columns of plane z + 4, close to when the neighbouring tiles fetch
+// the same cells as their own centre, so more of them come from L2.
+template <typename T, int TY, bool FIRST, bool LAST, bool DIAG, bool MT, int HD = 1>
 __global__ void __launch_bounds__(64 * TY)
     gr_stage_tiled(const GrK<T> k, const GrIO<T> io, const T As, const T Bs, const TileMap tm) {
+  static_assert(HD >= 1 && (!MT || HD == 1), "halo look-ahead");
   constexpr int TX = 64, R = 2, LX = TX + 2 * R, LY = TY + 2 * R, NL = 11, NT = TX * TY;
   constexpr int H = 2 * R * (TX + TY);
   constexpr int HPT = (H + NT - 1) / NT;
@@ -172,15 +177,16 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
     for (int f = 0; f < 7; f++) inc[f] = ldf<T, O32>(io.in[f], col + ak * k.sz);
   }
-  T hv[HPT][7];
-  CksT<T> hm[HPT];  // halo metric (MT)
-  {
-    const int ak = amap(z0, k.n[2], k.guard[2], 0);
+  T hq[HD][HPT][7];  // halo state of planes z .. z + HD - 1
+  CksT<T> hm[HPT];   // halo metric (MT)
+#pragma unroll
+  for (int d = 0; d < HD; d++) {
+    const int ak = amap(z0 + d, k.n[2], k.guard[2], 0);
 #pragma unroll
     for (int h = 0; h < HPT; h++)
       if (hok[h]) {
 #pragma unroll
-        for (int f = 0; f < 7; f++) hv[h][f] = ldf<T, O32>(io.in[f], hcol[h] + ak * k.sz);
+        for (int f = 0; f < 7; f++) hq[d][h][f] = ldf<T, O32>(io.in[f], hcol[h] + ak * k.sz);
         if constexpr (MT) hm[h] = gr_metric_load(io, hcol[h] + ak * k.sz);
       }
   }
@@ -228,17 +234,23 @@ __global__ void __launch_bounds__(64 * TY)
         if constexpr (MT) m = hm[h];
         else m = gr_metric_at(k, hai[h], haj[h], akz);
         T v[11];
-        gr_point_full(m, hv[h], v);
+        gr_point_full(m, hq[0][h], v);
 #pragma unroll
         for (int f = 0; f < NL; f++) lds[buf][f][hslot[h]] = v[f];
       }
-    if (z + 1 < z1) {
-      const int off = (akz + 1) * k.sz;
+#pragma unroll
+    for (int d = 0; d + 1 < HD; d++)
+#pragma unroll
+      for (int h = 0; h < HPT; h++)
+#pragma unroll
+        for (int f = 0; f < 7; f++) hq[d][h][f] = hq[d + 1][h][f];
+    if (z + HD < z1) {
+      const int off = (akz + HD) * k.sz;
 #pragma unroll
       for (int h = 0; h < HPT; h++)
         if (hok[h]) {
 #pragma unroll
-          for (int f = 0; f < 7; f++) hv[h][f] = ldf<T, O32>(io.in[f], hcol[h] + off);
+          for (int f = 0; f < 7; f++) hq[HD - 1][h][f] = ldf<T, O32>(io.in[f], hcol[h] + off);
           if constexpr (MT) hm[h] = gr_metric_load(io, hcol[h] + off);
         }
     }

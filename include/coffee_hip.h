@@ -207,7 +207,9 @@ int coffee_set_diagnostics(coffee_ctx *ctx, int enable);
  * array is below 4 GiB, variant 5 is the same tile with 64-bit addressing;
  * the GR solver runs its tiled schedule, every metric point computed on the
  * fly, for every variant but 1 (point sweep + stage sweep) and 2 (the halo
- * points' metric in stages 2-5 read from a table filled once); variants 1
+ * points' metric in stages 2-5 read from a table filled once); in FP32 its
+ * middle stages load the x-y halo three planes ahead except in variant 3
+ * (one plane ahead); variants 1
  * and 2 allocate 12 grid-sized scratch arrays on first use, and GR arrays of
  * 4 GiB or more always take the point schedule; zchunk = z planes per
  * workgroup of the tiled kernels, 0 = automatic.
