@@ -143,16 +143,18 @@ def test_gr_rejects_bad_spin():
 def test_gr_schedules_bitwise_identical(dtype, name):
     """The tiled stage kernel (E, H on the fly) equals the point + stage
     sweeps bitwise, with the halo metric computed on the fly (default) or
-    read from the table (variant 2), with z split into chunks and with
-    diagnostics."""
+    read from the table (variant 2), with the halo loaded one plane ahead
+    (variant 3), with z split into chunks and with diagnostics."""
     sp, U0 = case_gr(dtype, **GR_CASES[name])
     simple, ds = run_gpu(sp, U0, dtype, 3, diag=True, variant=1)
     tiled, dt_ = run_gpu(sp, U0, dtype, 3, diag=True, variant=0)
     chunked, _ = run_gpu(sp, U0, dtype, 3, variant=0, zchunk=5)
     table, dm = run_gpu(sp, U0, dtype, 3, diag=True, variant=2)
+    near, _ = run_gpu(sp, U0, dtype, 3, variant=3)  # halo one plane ahead
     assert np.array_equal(simple, tiled)
     assert np.array_equal(simple, chunked)
     assert np.array_equal(simple, table)
+    assert np.array_equal(simple, near)
     for key in ("J", "divB", "divE"):
         assert np.array_equal(ds[key], dt_[key])
         assert np.array_equal(ds[key], dm[key])
