@@ -141,10 +141,8 @@ __device__ __forceinline__ CksT<T> cks_scalar(const Cks& m) {
 }
 
 // The metric the solver kernels use at a point: Newton-refined FP64 in the
-// FP32 solver's stages, IEEE FP64 in its KO (where the refined sequences
-// raised the register count from 118 to 188 and halved the occupancy) and
-// everywhere in the FP64 solver.  Every schedule of a kernel uses the same
-// choice, so the schedules stay bitwise identical.
+// FP32 solver, IEEE FP64 in the FP64 solver.  Every schedule of a kernel
+// uses the same choice, so the schedules stay bitwise identical.
 template <typename T, bool FAST = sizeof(T) == 4>
 __device__ __forceinline__ CksT<T> gr_metric(double a, double x, double y, double z) {
   return cks_scalar<T, FAST>(cks_point<FAST>(a, x, y, z));
@@ -488,7 +486,7 @@ __global__ void __launch_bounds__(256) gr_ko_kernel(const GrK<T> k, const GrIO<T
     const T t = d6(p, c, ox) + d6(p, c, oy) + d6(p, c, oz);
     u[f] = fma(t, k.KOfac, p[c]);
   }
-  const CksT<T> m = gr_metric<T, false>(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak));
+  const CksT<T> m = gr_metric<T>(k.a, gr_pos(k, 0, ai), gr_pos(k, 1, aj), gr_pos(k, 2, ak));
   gr_project(k, m, u);
 #pragma unroll
   for (int f = 0; f < 7; f++) io.out[f][c] = u[f];

@@ -389,6 +389,8 @@ __global__ void __launch_bounds__(64 * TY)
 #pragma unroll
           for (int f = 0; f < NF; f++) hv[h][f] = io.in[f][hcol[h] + off];
     }
+    // the cell's metric while the plane's loads and the barrier are pending
+    const CksT<T> m = gr_metric_at<T>(k, ai, aj, az);
     __syncthreads();
     if (active) {
       T u[7];
@@ -400,7 +402,6 @@ __global__ void __launch_bounds__(64 * TY)
         const T kz = d6v(q[0][f], q[1][f], q[2][f], q[3][f], q[4][f], q[5][f], q[6][f]);
         u[f] = fma(kx + ky + kz, k.KOfac, q[R][f]);
       }
-      const CksT<T> m = gr_metric_at<T, false>(k, ai, aj, az);
       gr_project(k, m, u);
       const int c = col + az * k.sz;
 #pragma unroll
