@@ -611,13 +611,24 @@ __device__ __forceinline__ void sph_ko_point(const SphRun<T>& R, int i, int j, i
     wy[q] = R.KO_geometry ? r3c * fabs(G.s[q]) : T(1);
   }
   const T iden = R.KO_geometry ? T(1) / gm : T(1);
+  // the delta^6 coefficients (1, -6, 15, -20, 15, -6, 1) times the tap
+  // weights, once per cell for the seven fields: one FMA per tap
+  constexpr T c6[7] = {T(1), T(-6), T(15), T(-20), T(15), T(-6), T(1)};
+  T cx[7], cy[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    cx[q] = c6[q] * wx[q];
+    cy[q] = c6[q] * wy[q];
+  }
   T u[7];
 #pragma unroll
   for (int f = 0; f < 7; f++) {
-    const T kx = d6v(wx[0] * S.x(f, -3), wx[1] * S.x(f, -2), wx[2] * S.x(f, -1),
-                     wx[3] * S.x(f, 0), wx[4] * S.x(f, 1), wx[5] * S.x(f, 2), wx[6] * S.x(f, 3));
-    const T ky = d6v(wy[0] * S.y(f, -3), wy[1] * S.y(f, -2), wy[2] * S.y(f, -1),
-                     wy[3] * S.x(f, 0), wy[4] * S.y(f, 1), wy[5] * S.y(f, 2), wy[6] * S.y(f, 3));
+    T kx = cx[0] * S.x(f, -3), ky = cy[0] * S.y(f, -3);
+#pragma unroll
+    for (int q = 1; q < 7; q++) {
+      kx = fma(cx[q], S.x(f, q - 3), kx);
+      ky = fma(cy[q], q == 3 ? S.x(f, 0) : S.y(f, q - 3), ky);
+    }
     u[f] = fma((kx + ky) * iden, R.KOfac, S.x(f, 0));
   }
   const T g1 = r * r, g2 = r * r, rs = r * sth, g3 = rs * rs;
@@ -698,7 +709,7 @@ struct SphKoCell {
 
 // Kreiss-Oliger as a theta march (as the stage).
 template <typename T, int TX, int NJ>
-__global__ void __launch_bounds__(TX)
+__global__ void __launch_bounds__(TX) __attribute__((amdgpu_waves_per_eu(TX >= 512 ? 4 : 1, 8)))
     sph_ko_march_kernel(SphRun<T>* __restrict__ runs, const SphItem* __restrict__ items,
                         int nitems, int src) {
   __shared__ T lds[2][7][TX + 6];
